@@ -1,0 +1,185 @@
+/*
+ * mshqc_hip.h -- C ABI of the MI355X (gfx950) Cholesky-vector hot path.
+ *
+ * One shared library, libmshqc_hip.so, replaces the OpenMP/Eigen loops that
+ * syahrulhidayat/mshqc runs inside its SCF and MP2 classes.  Every entry point
+ * takes plain pointers and sizes; no C++ or torch types cross this boundary.
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers (hipMalloc'd, e.g. through
+ *     mshqc_hip_alloc, or torch tensors' data_ptr()), C order, FP64:
+ *       L  [Q][n][n]   Cholesky vectors L^Q_{mu nu} (symmetric in mu,nu)
+ *       P  [nspin][n][n]  densities;  Y [n][m] density factor (P = Y Y^T)
+ *       C  [n][m]      MO coefficient block (columns = orbitals)
+ *       B  [Q][o][v]   B^Q_{ia};  dEdB has the same layout
+ *       t2 [o][o][v][v]
+ *     Scalar energies are written to HOST pointers.
+ *   - Calls are synchronous: work is queued on the context's stream and the
+ *     stream is synchronised before return.  A context is not re-entrant;
+ *     use one context per host thread.
+ *   - Every call returns MSHQC_HIP_OK (0) or an error code; the message is
+ *     available from mshqc_hip_last_error().  No C++ exception crosses the ABI.
+ *   - Reductions run in a fixed order: results are bit-reproducible for a
+ *     given device, shape and precision.
+ *   - precision: MSHQC_PREC_FP64 (0) computes in FP64 on FP64 MFMA.  A value
+ *     k >= 2 selects the split-BF16 form with k BF16 slices per operand
+ *     (k = 2 is the hi+lo three-product form); slice products with
+ *     p + q <= k - 1 are kept, accumulated exactly in FP32 within 64-wide Q
+ *     blocks and summed across blocks in FP64.  Denominators and every
+ *     energy/gradient sum stay FP64.
+ */
+#ifndef MSHQC_HIP_H
+#define MSHQC_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSHQC_HIP_OK 0
+#define MSHQC_HIP_ERR_ARG 1
+#define MSHQC_HIP_ERR_DEVICE 2
+#define MSHQC_HIP_ERR_ALLOC 3
+#define MSHQC_HIP_ERR_UNSUPPORTED 4
+
+#define MSHQC_PREC_FP64 0
+
+typedef struct mshqc_hip_ctx mshqc_hip_ctx;
+
+/* ------------------------------------------------------------ context */
+
+/* Create a context on `device` with its own stream. */
+int mshqc_hip_ctx_create(int device, mshqc_hip_ctx** out);
+/* Destroy a context, its stream and its workspace.  NULL is accepted. */
+int mshqc_hip_ctx_destroy(mshqc_hip_ctx* ctx);
+/* Message of the last failed call on `ctx` ("" if none). */
+const char* mshqc_hip_last_error(const mshqc_hip_ctx* ctx);
+/* Queue subsequent work on `stream` (a hipStream_t); NULL restores the
+ * context's own stream. */
+int mshqc_hip_set_stream(mshqc_hip_ctx* ctx, void* stream);
+/* Device time, in ms, of the last compute call (HIP events around its kernels). */
+int mshqc_hip_last_ms(const mshqc_hip_ctx* ctx, float* ms);
+/* Cap, in doubles, of the per-occupied-block amplitude workspace used by the
+ * gradient calls (default 2^27 = 1 GiB).  Smaller caps mean more, smaller
+ * occupied blocks; results do not depend on it beyond rounding. */
+int mshqc_hip_set_zblock(mshqc_hip_ctx* ctx, size_t elems);
+/* Per-kernel-class timing (HIP events around each launch, on the stream the
+ * kernel runs on).  kind: 0 = pair-block (ia|jb) kernels, 1 = GEMM kernels
+ * (gradient contractions, K, transforms), 2 = J streaming kernels.
+ * mshqc_hip_profile(ctx, 1) clears and enables the counters, 0 disables. */
+int mshqc_hip_profile(mshqc_hip_ctx* ctx, int enable);
+int mshqc_hip_profile_get(const mshqc_hip_ctx* ctx, int kind, long* launches, double* total_ms);
+/* Library build tag, e.g. "mshqc_hip gfx950 <date>". */
+const char* mshqc_hip_version(void);
+
+/* ------------------------------------------------------------- memory */
+
+int mshqc_hip_alloc(mshqc_hip_ctx* ctx, size_t bytes, void** dptr);
+int mshqc_hip_free(mshqc_hip_ctx* ctx, void* dptr);
+int mshqc_hip_memcpy_h2d(mshqc_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int mshqc_hip_memcpy_d2h(mshqc_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* --------------------------------------------------------- Fock pieces */
+
+/*
+ * Coulomb and exchange from Cholesky vectors with general densities.
+ * Replaces CholeskyRHF::build_fock (src/scf/cholesky_rhf.cc:99-137) and
+ * CholeskyUHF::build_fock_cholesky (src/scf/cholesky_uhf.cc:77-103):
+ *   J      = sum_Q tr(L^Q P_tot) L^Q,   P_tot = sum_s P_s
+ *   K[s]   = sum_Q L^Q P_s L^Q
+ * screen != 0 skips K for vectors with ||L^Q||_F <= 1e-8 (the RHF
+ * `screen_exchange` option, cholesky_rhf.cc:120-126).  The caller assembles
+ * F = H + J - K/2 (RHF) or F_s = H + J - K[s] (UHF).
+ */
+int mshqc_hip_jk(mshqc_hip_ctx* ctx, const double* L, int Q, int n,
+                 const double* P, int nspin, int screen, double* J, double* K);
+
+/*
+ * Coulomb and exchange for a factored density P = Y Y^T (Y is n x m):
+ *   J = sum_Q tr(L^Q P) L^Q,   K = sum_Q (L^Q Y)(L^Q Y)^T
+ * The occupied-factor exchange of CholeskyROHF (src/scf/cholesky_rohf.cc:
+ * 125-133); with Y = sqrt(2) C_occ it equals the RHF K of an idempotent
+ * density (cholesky_rhf.cc:90-94,127).
+ */
+int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n,
+                        const double* Y, int m, double* J, double* K);
+
+/* ---------------------------------------------------------- transforms */
+
+/* Half transform X^Q = L^Q C  (X: [Q][n][m]). */
+int mshqc_hip_half_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n,
+                             const double* C, int m, double* X);
+
+/*
+ * Full transform B^Q = C_occ^T L^Q C_vir  (B: [Q][o][v]).
+ * Replaces CholeskyRMP2::transform_vectors (src/mp2/cholesky_rmp2.cc:68-92)
+ * and CholeskyUMP2::transform_cholesky_vectors (src/mp2/cholesky_ump2.cc:
+ * 244-293, called once per spin).  Computed occupied-first.
+ */
+int mshqc_hip_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n,
+                        const double* Cocc, int o, const double* Cvir, int v,
+                        double* B);
+
+/* ---------------------------------------------------------------- RMP2 */
+
+/*
+ * CD-RMP2 correlation energy, replacing the pair loop of
+ * CholeskyRMP2::compute (src/mp2/cholesky_rmp2.cc:118-155):
+ *   E = sum_{ij} sum_{ab} (ia|jb) [2 (ia|jb) - (ib|ja)] / D_ij^ab,
+ *   (ia|jb) = sum_Q B^Q_ia B^Q_jb,  D = e_i + e_j - e_a - e_b,
+ * terms with |D| < 1e-12 skipped.  (ia|jb) never leaves the chip.
+ */
+int mshqc_hip_rmp2_energy(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v,
+                          const double* eps_o, const double* eps_v, int precision,
+                          double* E);
+
+/*
+ * Energy and gradient (no reference counterpart: the backward of the energy
+ * above).  With t = (ia|jb)/D and T = 2t - t^T:
+ *   dEdB^Q_ia   = 4 Gamma^Q_ia,  Gamma^Q_ia = sum_{jb} T_ij^ab B^Q_jb
+ *   dE_deps_o_i = -2 sum_{jab} t T,   dE_deps_v_a = +2 sum_{ijb} t T
+ * Amplitudes are recomputed per occupied pair block, never stored whole.
+ * Any of dEdB / dE_deps_o / dE_deps_v may be NULL.
+ */
+int mshqc_hip_rmp2_grad(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v,
+                        const double* eps_o, const double* eps_v, int precision,
+                        double* E, double* dEdB, double* dE_deps_o, double* dE_deps_v);
+
+/*
+ * Amplitudes t2[i][j][a][b] = (ia|jb)/D (0 where |D| < 1e-12): the t2 tensor
+ * CholeskyRMP2 stores for CholeskyRMP3 (cholesky_rmp2.cc:145-148,160).
+ */
+int mshqc_hip_rmp2_t2(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v,
+                      const double* eps_o, const double* eps_v, double* t2);
+
+/* ---------------------------------------------------------------- UMP2 */
+
+/*
+ * CD-UMP2 energy components, replacing CholeskyUMP2::compute_ss_alpha,
+ * compute_ss_beta and compute_os (src/mp2/cholesky_ump2.cc:299-465):
+ *   E[0] = 1/4 sum_{ijab in aa} (g_ijab - g_ijba)^2 / D
+ *   E[1] = the same for beta,  E[2] = sum_{i in a, j in b, ab} g^2 / D
+ */
+int mshqc_hip_ump2_energy(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
+                          int Q, int oa, int va, int ob, int vb,
+                          const double* eps_oa, const double* eps_va,
+                          const double* eps_ob, const double* eps_vb,
+                          int precision, double* E);
+
+/* Energy components plus gradients with respect to Ba, Bb and the orbital
+ * energies of each spin (any gradient pointer may be NULL). */
+int mshqc_hip_ump2_grad(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
+                        int Q, int oa, int va, int ob, int vb,
+                        const double* eps_oa, const double* eps_va,
+                        const double* eps_ob, const double* eps_vb,
+                        int precision, double* E,
+                        double* dEdBa, double* dEdBb,
+                        double* dE_deps_oa, double* dE_deps_va,
+                        double* dE_deps_ob, double* dE_deps_vb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSHQC_HIP_H */

@@ -1,0 +1,97 @@
+"""Build recipe for the native parts of mshqc_amd (in-tree, no JIT cache).
+
+* ``libmshqc_hip.so``  -- HIP kernels + C ABI (hipcc, gfx950 only)
+* ``_mshqc*.so``       -- pybind11 module with the drop-in C++ classes
+                          (g++, links libmshqc_hip.so with an $ORIGIN rpath)
+
+Run ``python -m mshqc_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import glob
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIP_DIR = os.path.join(HERE, "csrc", "hip")
+HOST_DIR = os.path.join(HERE, "csrc", "host")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+HIP_LIB = os.path.join(HERE, "libmshqc_hip.so")
+
+HIPCC_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    # Keep hipcc from emitting packed FP32 instructions on its own.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def build_hip(jobs: int = 8) -> str:
+    """Compile every .hip source to an object in parallel, then link."""
+    srcs = sorted(glob.glob(os.path.join(HIP_DIR, "*.hip")))
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    hipcc = os.path.join(ROCM, "bin", "hipcc")
+    procs = []
+    objs = []
+    for s in srcs:
+        o = os.path.join(objdir, os.path.basename(s) + ".o")
+        objs.append(o)
+        cmd = [hipcc, *HIPCC_FLAGS, "-c", s, "-o", o]
+        print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+        if len(procs) >= jobs:
+            if procs.pop(0).wait() != 0:
+                raise RuntimeError("hipcc failed")
+    for p in procs:
+        if p.wait() != 0:
+            raise RuntimeError("hipcc failed")
+    tmp = HIP_LIB + ".tmp"
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, HIP_LIB)
+    return HIP_LIB
+
+
+def _ext_suffix() -> str:
+    return sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+
+
+def build_host() -> str:
+    """Compile the pybind11 module `_mshqc` against libmshqc_hip.so."""
+    import pybind11
+
+    srcs = sorted(glob.glob(os.path.join(HOST_DIR, "*.cc")))
+    if not srcs:
+        return ""
+    out = os.path.join(HERE, "_mshqc" + _ext_suffix())
+    tmp = out + ".tmp"
+    cmd = [
+        "g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall",
+        "-Wno-unused-function",
+        f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}",
+        f"-I{os.path.join(ROOT, 'include')}", f"-I{HOST_DIR}",
+        *srcs, "-o", tmp,
+        f"-L{HERE}", "-l:libmshqc_hip.so", "-Wl,-rpath,$ORIGIN",
+    ]
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    build_hip()
+    build_host()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    {"hip": build_hip, "host": build_host, "all": build_all}[what]()

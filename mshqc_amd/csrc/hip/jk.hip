@@ -1,0 +1,344 @@
+// Fock-build pieces (J, K) and AO->MO transforms from Cholesky vectors.
+//
+//   J   : two streaming passes over L (HBM-bound): gamma_Q = <L^Q, P_tot> with
+//         wavefront-shuffle reductions, then J = sum_Q gamma_Q L^Q.
+//   K   : FP64 MFMA GEMMs, batched over Q (general density: T = P L^Q then
+//         K += L^Q T; factored density: X^Q = L^Q Y then K += X^Q X^Q^T).
+//   B   : occupied-first transform X^Q = L^Q C_occ, B^Q = X^Q^T C_vir.
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+
+namespace {
+
+// One workgroup per Q: out[q] = sum_k L[q][k] * W[k] (W = P_tot), or the
+// squared Frobenius norm when W is null.  Fixed per-thread strides and a
+// fixed shuffle/LDS tree, so the result is reproducible.
+__global__ __launch_bounds__(256) void dot_per_q_kernel(const double* __restrict__ L, long nn,
+                                                        const double* __restrict__ W,
+                                                        double* __restrict__ out) {
+    const double* Lq = L + (long)blockIdx.x * nn;
+    double s = 0.0;
+    const long n2 = nn / 2;
+    const double2* L2 = reinterpret_cast<const double2*>(Lq);
+    const bool aligned = (((uintptr_t)Lq) & 15) == 0 && (W == nullptr || (((uintptr_t)W) & 15) == 0);
+    if (aligned) {
+        const double2* W2 = reinterpret_cast<const double2*>(W);
+        for (long k = threadIdx.x; k < n2; k += 256) {
+            const double2 l = L2[k];
+            if (W) {
+                const double2 w = W2[k];
+                s += l.x * w.x + l.y * w.y;
+            } else {
+                s += l.x * l.x + l.y * l.y;
+            }
+        }
+        for (long k = 2 * n2 + threadIdx.x; k < nn; k += 256) s += W ? Lq[k] * W[k] : Lq[k] * Lq[k];
+    } else {
+        for (long k = threadIdx.x; k < nn; k += 256) s += W ? Lq[k] * W[k] : Lq[k] * Lq[k];
+    }
+    for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// J[k] = sum_q gamma[q] L[q][k]; two consecutive k per thread.
+__global__ __launch_bounds__(256) void j_accumulate_kernel(const double* __restrict__ L, int Q,
+                                                           long nn, const double* __restrict__ gam,
+                                                           double* __restrict__ J) {
+    const long k = 2 * ((long)blockIdx.x * 256 + threadIdx.x);
+    if (k >= nn) return;
+    double s0 = 0.0, s1 = 0.0;
+    if (k + 1 < nn) {
+        for (int q = 0; q < Q; ++q) {
+            const double g = gam[q];
+            const double* p = L + (long)q * nn + k;
+            s0 += g * p[0];
+            s1 += g * p[1];
+        }
+        J[k] = s0;
+        J[k + 1] = s1;
+    } else {
+        for (int q = 0; q < Q; ++q) s0 += gam[q] * L[(long)q * nn + k];
+        J[k] = s0;
+    }
+}
+
+__global__ void sum_spins_kernel(const double* __restrict__ P, int nspin, long nn,
+                                 double* __restrict__ Pt) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    double s = 0.0;
+    for (int sp = 0; sp < nspin; ++sp) s += P[(long)sp * nn + k];
+    Pt[k] = s;
+}
+
+// Zero T_q for screened vectors (norm2[q] <= thr^2): their K contribution
+// then vanishes exactly.
+__global__ void mask_zero_kernel(double* __restrict__ T, const double* __restrict__ norm2, int q0,
+                                 int nq, long nn, double thr2) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nq * nn) return;
+    const int q = (int)(idx / nn);
+    if (!(norm2[q0 + q] > thr2)) T[idx] = 0.0;
+}
+
+int j_build(mshqc_hip_ctx* ctx, const double* L, int Q, long nn, const double* Pt, double* J,
+            double* gam) {
+    ProfScope ps(ctx, PROF_J);
+    dot_per_q_kernel<<<Q, 256, 0, ctx->stream>>>(L, nn, Pt, gam);
+    LAUNCH_CHECK(ctx);
+    const long nthr = (nn + 1) / 2;
+    j_accumulate_kernel<<<(unsigned)((nthr + 255) / 256), 256, 0, ctx->stream>>>(L, Q, nn, gam, J);
+    LAUNCH_CHECK(ctx);
+    return MSHQC_HIP_OK;
+}
+
+// Largest Q chunk whose n x m temporaries fit in `cap` doubles.
+int q_chunk(int Q, long per_q, size_t cap) {
+    long c = (long)(cap / (size_t)std::max(per_q, 1L));
+    return (int)std::max(1L, std::min((long)Q, c));
+}
+
+constexpr size_t kTmpCap = size_t(1) << 28;  // doubles (2 GiB) per temporary
+
+int gemm_splits(int M, int N, int batch) {
+    const long tiles = (long)((M + 63) / 64) * ((N + 63) / 64);
+    long s = (2048 + tiles - 1) / tiles;
+    return (int)std::max(1L, std::min((long)batch, s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int mshqc_hip_jk(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* P, int nspin,
+                 int screen, double* J, double* K) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, L && P && J && K, "jk: null pointer");
+    CHECK_ARG(ctx, Q > 0 && n > 0 && nspin >= 1 && nspin <= 2, "jk: bad dimensions");
+    hipEventRecord(ctx->ev0, ctx->stream);
+    const long nn = (long)n * n;
+    const int qc = q_chunk(Q, nn, kTmpCap);
+    double* tmp = static_cast<double*>(
+        workspace(ctx, WS_TMP, sizeof(double) * ((size_t)qc * nn + nn + 2 * (size_t)Q + 64)));
+    if (!tmp) return MSHQC_HIP_ERR_ALLOC;
+    double* T = tmp;
+    double* Pt = tmp + (size_t)qc * nn;
+    double* gam = Pt + nn;
+    double* norm2 = gam + Q;
+    sum_spins_kernel<<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(P, nspin, nn, Pt);
+    LAUNCH_CHECK(ctx);
+    if ((rc = j_build(ctx, L, Q, nn, Pt, J, gam))) return rc;
+    if (screen) {
+        dot_per_q_kernel<<<Q, 256, 0, ctx->stream>>>(L, nn, nullptr, norm2);
+        LAUNCH_CHECK(ctx);
+    }
+    for (int s = 0; s < nspin; ++s) {
+        double* Ks = K + (long)s * nn;
+        for (int q0 = 0; q0 < Q; q0 += qc) {
+            const int nq = std::min(qc, Q - q0);
+            // T_q = P_s L^q
+            GemmF64 g1;
+            g1.A = P + (long)s * nn;
+            g1.lda = n;
+            g1.sA = 0;
+            g1.X = L + (long)q0 * nn;
+            g1.ldx = n;
+            g1.sX = nn;
+            g1.C = T;
+            g1.ldc = n;
+            g1.sC = nn;
+            g1.M = g1.N = g1.K = n;
+            g1.batch = nq;
+            { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g1); }
+            if (rc) return rc;
+            if (screen) {
+                const long tot = (long)nq * nn;
+                mask_zero_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(
+                    T, norm2, q0, nq, nn, 1e-16);
+                LAUNCH_CHECK(ctx);
+            }
+            // K_s (+)= sum_q L^q T_q
+            GemmF64 g2;
+            g2.A = L + (long)q0 * nn;
+            g2.lda = n;
+            g2.sA = nn;
+            g2.X = T;
+            g2.ldx = n;
+            g2.sX = nn;
+            g2.C = Ks;
+            g2.ldc = n;
+            g2.M = g2.N = g2.K = n;
+            g2.batch = nq;
+            g2.reduce_batch = true;
+            g2.splits = gemm_splits(n, n, nq);
+            g2.accumulate = q0 > 0;
+            { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
+            if (rc) return rc;
+        }
+    }
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if ((rc = ctx_finish(ctx))) return rc;
+    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+    return MSHQC_HIP_OK;
+}
+
+int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* Y, int m,
+                        double* J, double* K) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, L && Y && J && K, "jk_factor: null pointer");
+    CHECK_ARG(ctx, Q > 0 && n > 0 && m > 0, "jk_factor: bad dimensions");
+    hipEventRecord(ctx->ev0, ctx->stream);
+    const long nn = (long)n * n, nm = (long)n * m;
+    const int qc = q_chunk(Q, nm, kTmpCap);
+    double* tmp = static_cast<double*>(
+        workspace(ctx, WS_TMP, sizeof(double) * ((size_t)qc * nm + nn + (size_t)Q + 64)));
+    if (!tmp) return MSHQC_HIP_ERR_ALLOC;
+    double* X = tmp;
+    double* P = tmp + (size_t)qc * nm;
+    double* gam = P + nn;
+    // P = Y Y^T for J
+    GemmF64 gp;
+    gp.A = Y;
+    gp.lda = m;
+    gp.X = Y;
+    gp.ldx = m;
+    gp.transX = true;
+    gp.C = P;
+    gp.ldc = n;
+    gp.M = gp.N = n;
+    gp.K = m;
+    if ((rc = gemm_f64(ctx, gp))) return rc;
+    if ((rc = j_build(ctx, L, Q, nn, P, J, gam))) return rc;
+    for (int q0 = 0; q0 < Q; q0 += qc) {
+        const int nq = std::min(qc, Q - q0);
+        // X_q = L^q Y
+        GemmF64 g1;
+        g1.A = L + (long)q0 * nn;
+        g1.lda = n;
+        g1.sA = nn;
+        g1.X = Y;
+        g1.ldx = m;
+        g1.sX = 0;
+        g1.C = X;
+        g1.ldc = m;
+        g1.sC = nm;
+        g1.M = n;
+        g1.N = m;
+        g1.K = n;
+        g1.batch = nq;
+        { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g1); }
+            if (rc) return rc;
+        // K (+)= sum_q X_q X_q^T
+        GemmF64 g2;
+        g2.A = X;
+        g2.lda = m;
+        g2.sA = nm;
+        g2.X = X;
+        g2.ldx = m;
+        g2.sX = nm;
+        g2.transX = true;
+        g2.C = K;
+        g2.ldc = n;
+        g2.M = g2.N = n;
+        g2.K = m;
+        g2.batch = nq;
+        g2.reduce_batch = true;
+        g2.splits = gemm_splits(n, n, nq);
+        g2.accumulate = q0 > 0;
+        { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
+            if (rc) return rc;
+    }
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if ((rc = ctx_finish(ctx))) return rc;
+    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+    return MSHQC_HIP_OK;
+}
+
+int mshqc_hip_half_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* C,
+                             int m, double* X) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, L && C && X, "half_transform: null pointer");
+    CHECK_ARG(ctx, Q > 0 && n > 0 && m > 0, "half_transform: bad dimensions");
+    hipEventRecord(ctx->ev0, ctx->stream);
+    GemmF64 g;
+    g.A = L;
+    g.lda = n;
+    g.sA = (long)n * n;
+    g.X = C;
+    g.ldx = m;
+    g.sX = 0;
+    g.C = X;
+    g.ldc = m;
+    g.sC = (long)n * m;
+    g.M = n;
+    g.N = m;
+    g.K = n;
+    g.batch = Q;
+    { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
+    if (rc) return rc;
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if ((rc = ctx_finish(ctx))) return rc;
+    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+    return MSHQC_HIP_OK;
+}
+
+int mshqc_hip_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* Cocc,
+                        int o, const double* Cvir, int v, double* B) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, L && Cocc && Cvir && B, "transform: null pointer");
+    CHECK_ARG(ctx, Q > 0 && n > 0 && o > 0 && v > 0, "transform: bad dimensions");
+    hipEventRecord(ctx->ev0, ctx->stream);
+    const long nn = (long)n * n, no = (long)n * o;
+    const int qc = q_chunk(Q, no, kTmpCap);
+    double* X = static_cast<double*>(workspace(ctx, WS_TMP, sizeof(double) * (size_t)qc * no));
+    if (!X) return MSHQC_HIP_ERR_ALLOC;
+    for (int q0 = 0; q0 < Q; q0 += qc) {
+        const int nq = std::min(qc, Q - q0);
+        GemmF64 g1;  // X_q = L^q C_occ  (n x o)
+        g1.A = L + (long)q0 * nn;
+        g1.lda = n;
+        g1.sA = nn;
+        g1.X = Cocc;
+        g1.ldx = o;
+        g1.C = X;
+        g1.ldc = o;
+        g1.sC = no;
+        g1.M = n;
+        g1.N = o;
+        g1.K = n;
+        g1.batch = nq;
+        { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g1); }
+            if (rc) return rc;
+        GemmF64 g2;  // B_q = X_q^T C_vir  (o x v)
+        g2.A = X;
+        g2.lda = o;
+        g2.sA = no;
+        g2.transA = true;
+        g2.X = Cvir;
+        g2.ldx = v;
+        g2.C = B + (long)q0 * o * v;
+        g2.ldc = v;
+        g2.sC = (long)o * v;
+        g2.M = o;
+        g2.N = v;
+        g2.K = n;
+        g2.batch = nq;
+        { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
+            if (rc) return rc;
+    }
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if ((rc = ctx_finish(ctx))) return rc;
+    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+    return MSHQC_HIP_OK;
+}
+
+}  // extern "C"

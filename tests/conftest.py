@@ -34,3 +34,18 @@ def to_dev(*arrays):
 
 def to_np(t):
     return t.detach().cpu().numpy()
+
+
+def check_close(got, ref, rtol, name="value", atol=0.0):
+    """Max-norm relative check with a short failure message (no array dumps)."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    if got.shape != ref.shape:
+        raise AssertionError(f"{name}: shape {got.shape} != {ref.shape}")
+    scale = float(np.abs(ref).max()) if ref.size else 0.0
+    err = float(np.abs(got - ref).max()) if ref.size else 0.0
+    if not (err <= rtol * scale + atol):
+        idx = np.unravel_index(np.argmax(np.abs(got - ref)), ref.shape) if ref.ndim else ()
+        raise AssertionError(f"{name}: max |diff| {err:.3e} > {rtol:g} * {scale:.3e} + {atol:g} "
+                             f"(worst at {idx})")
+    return err

@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import to_dev, to_np
+from conftest import check_close, to_dev, to_np
 from mshqc_amd import synthetic as syn
 
 pytestmark = pytest.mark.gpu
@@ -22,10 +22,10 @@ def test_transform_matches_oracle(ctx, Q, n, o, seed):
     B_ref = oracle.transform(L, Co, Cv)
     B = to_np(ctx.transform(*to_dev(L, Co, Cv)))
     print(f"transform Q={Q} n={n} o={o}", flush=True)
-    assert np.abs(B - B_ref).max() <= 1e-12 * np.abs(B_ref).max()
+    check_close(B, B_ref, 1e-12, "B_ref")
     X_ref = oracle.half_transform(L, Co)
     X = to_np(ctx.half_transform(*to_dev(L, Co)))
-    assert np.abs(X - X_ref).max() <= 1e-12 * np.abs(X_ref).max()
+    check_close(X, X_ref, 1e-12, "X_ref")
 
 
 @pytest.mark.parametrize("Q,n,nspin,screen,seed", [(23, 37, 1, True, 4), (31, 50, 2, False, 5),
@@ -42,8 +42,8 @@ def test_jk_general_density_matches_oracle(ctx, Q, n, nspin, screen, seed):
     J_ref, K_ref = oracle.jk(L, P_arg, screen=screen)
     J, K = ctx.jk(*to_dev(L, P_arg), screen=screen)
     print(f"jk Q={Q} n={n} nspin={nspin} screen={screen}", flush=True)
-    assert np.abs(to_np(J) - J_ref).max() <= 1e-12 * np.abs(J_ref).max()
-    assert np.abs(to_np(K) - K_ref).max() <= 1e-12 * np.abs(K_ref).max()
+    check_close(to_np(J), J_ref, 1e-12, "J_ref")
+    check_close(to_np(K), K_ref, 1e-12, "K_ref")
 
 
 def test_jk_factor_matches_general(ctx):
@@ -54,5 +54,5 @@ def test_jk_factor_matches_general(ctx):
     Y = np.sqrt(2.0) * C
     J_ref, K_ref = oracle.jk(L, Y @ Y.T)
     J, K = ctx.jk_factor(*to_dev(L, Y))
-    assert np.abs(to_np(J) - J_ref).max() <= 1e-12 * np.abs(J_ref).max()
-    assert np.abs(to_np(K) - K_ref).max() <= 1e-12 * np.abs(K_ref).max()
+    check_close(to_np(J), J_ref, 1e-12, "J_ref")
+    check_close(to_np(K), K_ref, 1e-12, "K_ref")

@@ -24,7 +24,7 @@ struct mshqc_hip_ctx {
         void* ptr = nullptr;
         size_t bytes = 0;
     };
-    Slab slabs[8];
+    Slab slabs[12];
     // Event pair for timing the last compute call (milliseconds).
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
@@ -59,8 +59,10 @@ enum WsSlot {
     WS_MISC = 3,   // pair/tile lists, scalars
     WS_SLICE = 4,  // split-BF16 operand slices
     WS_SLICE2 = 5,
-    WS_TMP = 6,    // GEMM temporaries (J/K, transforms)
+    WS_TMP = 6,    // GEMM temporaries (J/K, transforms), energy slots
     WS_TMP2 = 7,
+    WS_ZS = 8,     // split-BF16 slices of the amplitude block
+    WS_ZTS = 9,    // ... and of its transpose
 };
 
 // Sets ctx->last_error and returns the status code.

@@ -9,19 +9,30 @@ constexpr int BF16_MAX_SLICES = 6;
 // Device-resident BF16 slices of the B operands of one MP2 problem.
 struct Bf16Operands {
     int slices = 0;
-    int Q = 0, qblocks = 0;
-    // Left and right operands: slices [k][Q][o][v] as bf16 bits and per
-    // (q-block, i, a) power-of-two exponents.
+    int Q = 0, Qp = 0, qblocks = 0;
+    int ol = 0, vl = 0, orr = 0, vr = 0, vlp = 0, vrp = 0;
+    // Pair-kernel operands: slices [S][o*v][Qp] (bf16 bits) and exponents
+    // [o*v][Qp/128], left and right (right = left in the symmetric modes).
     const uint16_t* sl = nullptr;
     const int8_t* xl = nullptr;
     const uint16_t* sr = nullptr;
     const int8_t* xr = nullptr;
+    // Gradient-GEMM operands: slices [S][Q][o*vp] with exponents per
+    // (q, i, 64-wide a block), left and right.
+    const uint16_t* gl = nullptr;
+    const int8_t* gel = nullptr;
+    const uint16_t* gr = nullptr;
+    const int8_t* ger = nullptr;
 };
 
-// Slice Bl (and Br when not null) into BF16 planes.
+// Slice Bl (and Br when not null) into BF16 planes; `grad` adds the GEMM layout.
 int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int vl,
-                 const double* Br, int orr, int vr, int Q, Bf16Operands* out);
+                 const double* Br, int orr, int vr, int Q, bool grad, Bf16Operands* out);
 // Pair-tile kernel on BF16 MFMA with the shared FP64 epilogue.
 int bf16_launch_pair(mshqc_hip_ctx* ctx, int mode, const PairArgs& a, const Bf16Operands& ops);
-// C += A * op(X) on BF16 MFMA with split operands (same contract as gemm_f64).
-int bf16_gemm(mshqc_hip_ctx* ctx, int slices, const GemmF64& g);
+// Gradient contractions of one occupied block pair from the FP64 amplitude
+// block Z (rows (i,a) of I, columns (j,b) of J):
+//   dBl[q][(i,a)] += sum Br[q][(j,b)] Z[(i,a)][(j,b)]
+//   dBr[q][(j,b)] += sum Bl[q][(i,a)] Z[(i,a)][(j,b)]   (when `second`)
+int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& ops, const double* Z, long ldz,
+                       int I0, int nI, int J0, int nJ, double* dBl, double* dBr, bool second);

@@ -207,7 +207,8 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
     Bf16Operands bops;
     if (pr.precision != 0) {
         int rc = bf16_prepare(ctx, pr.precision, pr.Bl, pr.ol, pr.vl,
-                              sym ? nullptr : pr.Br, pr.orr, pr.vr, pr.Q, &bops);
+                              sym ? nullptr : pr.Br, pr.orr, pr.vr, pr.Q,
+                              pr.grad && (pr.dBl || pr.dBr), &bops);
         if (rc) return rc;
     }
 
@@ -272,6 +273,14 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
             LAUNCH_CHECK(ctx);
         }
         if (!Z) continue;
+        const bool second = !sym || sg.I0 != sg.J0;
+        if (pr.precision != 0) {
+            ProfScope ps(ctx, PROF_GEMM);
+            rc = bf16_grad_contract(ctx, bops, Z, ldz, sg.I0, sg.nI, sg.J0, sg.nJ, pr.dBl,
+                                    pr.dBr, second);
+            if (rc) return rc;
+            continue;
+        }
         // dBl[q][(i,a) in I] += sum_{(j,b) in J} Br[q][(j,b)] Z[(i,a)][(j,b)]
         if (pr.dBl) {
             GemmF64 g;
@@ -289,12 +298,11 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
             g.accumulate = true;
             {
                 ProfScope ps(ctx, PROF_GEMM);
-                rc = pr.precision == 0 ? gemm_f64(ctx, g) : bf16_gemm(ctx, pr.precision, g);
+                rc = gemm_f64(ctx, g);
             }
             if (rc) return rc;
         }
         // dBr[q][(j,b) in J] += sum_{(i,a) in I} Bl[q][(i,a)] Z[(i,a)][(j,b)]
-        const bool second = !sym || sg.I0 != sg.J0;
         if (second && pr.dBr) {
             GemmF64 g;
             g.A = pr.Bl + (long)sg.I0 * pr.vl;
@@ -311,7 +319,7 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
             g.accumulate = true;
             {
                 ProfScope ps(ctx, PROF_GEMM);
-                rc = pr.precision == 0 ? gemm_f64(ctx, g) : bf16_gemm(ctx, pr.precision, g);
+                rc = gemm_f64(ctx, g);
             }
             if (rc) return rc;
         }

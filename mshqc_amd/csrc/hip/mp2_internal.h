@@ -20,14 +20,23 @@ __device__ __forceinline__ double xor_sum(double v, int lo_mask, int hi_mask) {
 // v_mfma 16x16 accumulator layout (wave tile 32x32 = 2x2 subtiles).
 // Writes the weighted tile partials (energy, and for gradients the dE/dD
 // sums by row/column) and, when requested, the gradient amplitudes z or t2.
-template <int MODE>
+// ROWMAP selects the accumulator row layout: 0 = FP64 MFMA (row = l/16 + 4r),
+// 1 = FP32 MFMA of the BF16 path (row = 4(l/16) + r).  Waves with
+// active == false only take part in the barrier (waves 0..3 own the tile).
+template <int MODE, int ROWMAP = 0>
 __device__ __forceinline__ void pair_epilogue(const PairArgs& p, int i, int j, double w, int A0,
                                               int B0, bool mirror, int lane, int wave, int wm,
                                               int wn, const v4d (&accG)[2][2],
                                               const v4d (&accH)[2][2], double (*red_row)[64],
                                               double (*red_col)[64], double* red_e, double* red_s,
-                                              long work) {
+                                              long work, bool active = true) {
+#pragma clang fp contract(off)
     const bool grad = p.s_part != nullptr;
+    auto rowof = [](int l, int r) { return ROWMAP ? 4 * (l >> 4) + r : (l >> 4) + 4 * r; };
+    if (!active) {
+        __syncthreads();
+        return;
+    }
     const double ei = p.eol[i] + p.eor[j];
     double e = 0.0, s = 0.0;
     double rowq[2][4], colq[2];
@@ -45,7 +54,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& p, int i, int j, d
         for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int a = A0 + wm + 16 * sa + mfma_row(lane, r);
+                const int a = A0 + wm + 16 * sa + rowof(lane, r);
                 const int b = B0 + wn + 16 * tb + mfma_col(lane);
                 if (a >= p.vl || b >= p.vr) continue;
                 const double D = ei - p.evl[a] - p.evr[b];
@@ -85,7 +94,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& p, int i, int j, d
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const double v = xor_sum(rowq[sa][r], 1, 16);
-                if ((lane & 15) == 0) red_row[wave & 1][wm + 16 * sa + mfma_row(lane, r)] = v;
+                if ((lane & 15) == 0) red_row[wave & 1][wm + 16 * sa + rowof(lane, r)] = v;
             }
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb) {

@@ -1,0 +1,67 @@
+// Split-BF16 arithmetic on gfx950 BF16 matrix cores (v_mfma_f32_16x16x32_bf16).
+//
+// Each FP64 operand element x of a block (one row over one block of the
+// contraction index) is written as
+//     x = 2^e * sum_{p < S} m_p * 2^{-8(p+1)}  (+ truncation < 2^{e-8S-1})
+// where 2^e bounds |x| over the block and every m_p is an integer with
+// |m_0| <= 256 and |m_p| <= 128 (p >= 1) -- exactly representable in BF16.
+// A slice product m_p * m_q is an exact integer below 2^16, so the products
+// of one class c = p + q, summed over a block of <= 128 contraction indices,
+// stay below 2^24 and are accumulated EXACTLY by the FP32 MFMA accumulator.
+// After every block the class accumulators are scaled by 2^{eA+eB-8(c+2)}
+// and added in FP64.  Products with p + q <= S-1 are kept (S(S+1)/2 MFMAs
+// per output tile instead of one FP64 MFMA).
+//
+// MFMA 16x16x32 bf16 lane layout (measured on gfx950 with one-hot operands):
+//   A: lane l holds A[row = l%16][k = 8*(l/16) + 0..7]
+//   B: lane l holds B[k = 8*(l/16) + 0..7][col = l%16]
+//   D: lane l, element r holds D[row = 4*(l/16) + r][col = l%16]
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ v4f mfma_bf16(const v8s& a, const v8s& b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                   __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int bf16_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+
+// Exponent e with |x| < 2^e for the block maximum `mx` (0 for an all-zero block).
+__device__ __forceinline__ int block_exponent(double mx) {
+    if (!(mx > 0.0)) return 0;
+    int e;
+    frexp(mx, &e);
+    return e;
+}
+
+// 2^k as a double for |k| < 1000.
+__device__ __forceinline__ double pow2(int k) {
+    return __longlong_as_double((long long)(k + 1023) << 52);
+}
+
+// BF16 bit pattern of an integer-valued float with |m| <= 256 (exact).
+__device__ __forceinline__ uint16_t bf16_bits_exact(float m) {
+    return (uint16_t)(__float_as_uint(m) >> 16);
+}
+
+// Slice x (|x| < 2^e) into S BF16 integers (bit patterns in out[p]).
+template <int S>
+__device__ __forceinline__ void slice_value(double x, int e, uint16_t (&out)[S]) {
+    double r = x * pow2(-e);  // |r| < 1 (exact scaling)
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        const double t = r * 256.0;
+        const double m = rint(t);
+        out[p] = bf16_bits_exact((float)m);
+        r = t - m;  // exact: |r| <= 1/2
+    }
+}
+
+// Number of kept products for S slices.
+__host__ __device__ constexpr int ozaki_products(int S) { return S * (S + 1) / 2; }

@@ -1,0 +1,89 @@
+"""Split-BF16 form (k BF16 slices per operand, exact FP32 class sums per Q
+block, FP64 across blocks) against the FP64 path and the oracle.
+
+Bars (relative, stated per slice count):
+  * inputs exactly representable in one slice -> (ia|jb) exact, energy equal
+    to the FP64 path's up to the last bits of the epilogue rounding
+  * |dE|/|E| <= 1e-4 (k=2, the hi+lo three-product form), 1e-8 (k=3),
+    1e-9 (k=4), 1e-12 (k=5), 1e-13 (k=6)
+  * gradient ||d dE/dB|| / ||dE/dB|| <= 1e-6 for k >= 3
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import check_close, to_dev, to_np
+from mshqc_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = {2: 1e-4, 3: 1e-8, 4: 1e-9, 5: 1e-12, 6: 1e-13}
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 6])
+def test_exact_inputs_match_fp64(ctx, k):
+    rng = np.random.default_rng(70 + k)
+    Q, o, v = 200, 4, 90
+    B = rng.integers(-255, 256, size=(Q, o, v)) / 256.0
+    _, eo, ev = syn.rmp2_inputs(1, o, v, 71)
+    dB, deo, dev = to_dev(B, eo, ev)
+    E64 = ctx.rmp2_energy(dB, deo, dev, precision=0)
+    Ek = ctx.rmp2_energy(dB, deo, dev, precision=k)
+    print(f"exact inputs k={k}: {Ek!r} vs {E64!r}", flush=True)
+    assert abs(Ek - E64) <= 2e-15 * abs(E64)
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 6])
+def test_energy_error_bound(ctx, k):
+    B, eo, ev = syn.rmp2_inputs(300, 6, 150, 80)
+    E_ref = oracle.rmp2_energy(B, eo, ev)
+    E = ctx.rmp2_energy(*to_dev(B, eo, ev), precision=k)
+    rel = abs(E - E_ref) / abs(E_ref)
+    print(f"k={k}: E {E!r} oracle {E_ref!r} rel {rel:.2e}", flush=True)
+    assert rel <= E_TOL[k]
+
+
+@pytest.mark.parametrize("k", [3, 5])
+def test_grad_error_bound(ctx, k):
+    B, eo, ev = syn.rmp2_inputs(200, 5, 130, 81)
+    E_ref, g_ref, deo_ref, dev_ref = oracle.rmp2_grad(B, eo, ev)
+    E, g, d_o, d_v = ctx.rmp2_grad(*to_dev(B, eo, ev), precision=k)
+    g = to_np(g)
+    rel = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
+    print(f"k={k}: grad rel {rel:.2e}, E rel {abs(E - E_ref) / abs(E_ref):.2e}", flush=True)
+    assert rel <= 1e-6
+    assert abs(E - E_ref) / abs(E_ref) <= E_TOL[k]
+    check_close(to_np(d_o), deo_ref, 1e-6, "dE/deps_o")
+    check_close(to_np(d_v), dev_ref, 1e-6, "dE/deps_v")
+
+
+def test_grad_blocks_bf16(ctx):
+    """Several occupied block pairs (ragged v) in the split-BF16 gradient."""
+    from mshqc_amd import hip
+
+    B, eo, ev = syn.rmp2_inputs(150, 7, 100, 82)
+    E_ref, g_ref, _, _ = oracle.rmp2_grad(B, eo, ev)
+    c = hip.Context(0)
+    try:
+        c.set_zblock(3 * 100 * 3 * 100)
+        E, g, _, _ = c.rmp2_grad(*to_dev(B, eo, ev), precision=5)
+    finally:
+        c.close()
+    rel = np.linalg.norm(to_np(g) - g_ref) / np.linalg.norm(g_ref)
+    assert rel <= 1e-9
+    assert abs(E - E_ref) / abs(E_ref) <= 1e-12
+
+
+def test_ump2_bf16(ctx):
+    (Ba, eoa, eva), (Bb, eob, evb) = syn.ump2_inputs(160, 4, 70, 3, 75, 83)
+    E_ref, grads_ref = oracle.ump2(Ba, Bb, eoa, eva, eob, evb, grad=True)
+    dev = to_dev(Ba, Bb, eoa, eva, eob, evb)
+    E = ctx.ump2_energy(*dev, precision=5)
+    for x, y in zip(E, E_ref):
+        assert abs(x - y) <= 1e-12 * abs(y)
+    E2, grads = ctx.ump2_grad(*dev, precision=5)
+    for x, y in zip(E2, E_ref):
+        assert abs(x - y) <= 1e-12 * abs(y)
+    for a, b in zip(grads[:2], grads_ref[:2]):
+        a = to_np(a)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-9

@@ -5,10 +5,14 @@
 
 // Largest supported slice count per operand.
 constexpr int BF16_MAX_SLICES = 6;
+// Slices of the gradient contractions: 3 keep the relative gradient error
+// near 1e-7, below the 1e-6 bar, at 6 instead of up to 21 BF16 products.
+constexpr int BF16_GRAD_SLICES = 3;
 
 // Device-resident BF16 slices of the B operands of one MP2 problem.
 struct Bf16Operands {
-    int slices = 0;
+    int slices = 0;   // slices of the (ia|jb) operands
+    int gslices = 0;  // slices of the gradient-GEMM operands
     int Q = 0, Qp = 0, qblocks = 0;
     int ol = 0, vl = 0, orr = 0, vr = 0, vlp = 0, vrp = 0;
     // Pair-kernel operands: slices [S][o*v][Qp] (bf16 bits) and exponents

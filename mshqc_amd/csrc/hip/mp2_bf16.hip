@@ -169,7 +169,7 @@ __device__ __forceinline__ v8s lds_frag(const char* panel, int row, int seg) {
 // columns from panel pb (each [S] slices of [64][32] at stride `sstride`).
 template <int S>
 __device__ __forceinline__ void chunk_mfma(const char* pa, const char* pb, long sstride, int wm,
-                                           int wn, int lane, v4f (&acc)[S][2][2]) {
+                                           int wn, int lane, v4f (&acc)[3][2][2]) {
     // All S column slices stay in registers; row slices are loaded one at a
     // time (slice p meets column slices q < S - p).
     v8s b[S][2];
@@ -189,13 +189,12 @@ __device__ __forceinline__ void chunk_mfma(const char* pa, const char* pb, long 
             for (int s = 0; s < 2; ++s)
 #pragma unroll
                 for (int t = 0; t < 2; ++t)
-                    acc[p + q][s][t] = mfma_bf16(a[s], b[q][t], acc[p + q][s][t]);
+                    acc[ozaki_tier(p, q)][s][t] = mfma_bf16(a[s], b[q][t], acc[ozaki_tier(p, q)][s][t]);
     }
 }
 
-// acc64 += sum_c acc32[c] * 2^(eA(row) + eB(col) - 8(c+2)); acc32 = 0.
-template <int S>
-__device__ __forceinline__ void flush(v4f (&acc)[S][2][2], v4d (&acc64)[2][2], const int (&eA)[2][4],
+// acc64 += (tier0 + tier1 + tier2) * 2^(eA(row) + eB(col) - 16); tiers = 0.
+__device__ __forceinline__ void flush(v4f (&acc)[3][2][2], v4d (&acc64)[2][2], const int (&eA)[2][4],
                                       const int (&eB)[2]) {
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -203,13 +202,12 @@ __device__ __forceinline__ void flush(v4f (&acc)[S][2][2], v4d (&acc64)[2][2], c
         for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                double sum = 0.0;
-#pragma unroll
-                for (int c = S - 1; c >= 0; --c) sum = sum * (1.0 / 256.0) + (double)acc[c][s][t][r];
+                const double sum = ((double)acc[0][s][t][r] + (double)acc[1][s][t][r]) +
+                                   (double)acc[2][s][t][r];
                 acc64[s][t][r] += sum * pow2(eA[s][r] + eB[t] - 16);
             }
 #pragma unroll
-    for (int c = 0; c < S; ++c)
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -284,7 +282,7 @@ __global__ __launch_bounds__(MODE == MODE_OS ? 256 : 512) void pair_bf16_kernel(
         }
     };
 
-    v4f acc[S][2][2];
+    v4f acc[3][2][2];
     v4d acc64[2][2];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -292,7 +290,7 @@ __global__ __launch_bounds__(MODE == MODE_OS ? 256 : 512) void pair_bf16_kernel(
         for (int t = 0; t < 2; ++t) {
             acc64[s][t] = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int c = 0; c < S; ++c) acc[c][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < 3; ++c) acc[c][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
         }
     // This wave's operand panels and exponent rows.
     const char* pa = lds + (isH ? 2 : 0) * PSTRIDE;
@@ -301,10 +299,8 @@ __global__ __launch_bounds__(MODE == MODE_OS ? 256 : 512) void pair_bf16_kernel(
     const int8_t* exB = isH ? Pel + ((long)i * p.vl) * Qb : Per + ((long)j * p.vr) * Qb;
     const int limA = isH ? p.vr : p.vl, limB = isH ? p.vl : p.vr;
 
-    // Register prefetch of the next chunk overlaps the loads with the MFMAs;
-    // above four slices the prefetch buffer would push the kernel into
-    // spills, so it loads synchronously instead.
-    constexpr bool PREFETCH = S <= 4;
+    // Register prefetch of the next chunk overlaps the loads with the MFMAs.
+    constexpr bool PREFETCH = true;
     const int nchunk = Qp / KC;
     if (PREFETCH) fetch(0);
     for (int ch = 0; ch < nchunk; ++ch) {
@@ -329,7 +325,7 @@ __global__ __launch_bounds__(MODE == MODE_OS ? 256 : 512) void pair_bf16_kernel(
                 const int b = B0 + wn + 16 * t + (lane & 15);
                 eB[t] = b < limB ? exB[(long)b * Qb + qb] : 0;
             }
-            flush<S>(acc, acc64, eA, eB);
+            flush(acc, acc64, eA, eB);
         }
     }
 
@@ -417,7 +413,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
             }
         }
     };
-    v4f acc[S][2][2];
+    v4f acc[3][2][2];
     v4d acc64[2][2];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -425,9 +421,9 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
         for (int t = 0; t < 2; ++t) {
             acc64[s][t] = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int c = 0; c < S; ++c) acc[c][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < 3; ++c) acc[c][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
         }
-    constexpr bool PREFETCH = S <= 4;
+    constexpr bool PREFETCH = true;
     const int nchunk = K / KC;
     if (PREFETCH) fetch(0);
     for (int ch = 0; ch < nchunk; ++ch) {
@@ -452,7 +448,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
                 const int n = n0 + wn + 16 * t + (lane & 15);
                 eB[t] = n < X.rows ? X.e[(long)n * X.lde + kb] : 0;
             }
-            flush<S>(acc, acc64, eA, eB);
+            flush(acc, acc64, eA, eB);
         }
     }
 #pragma unroll
@@ -580,6 +576,7 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     by_slices<AllowLds>(S, ctx);
     Bf16Operands o;
     o.slices = S;
+    o.gslices = std::min(S, BF16_GRAD_SLICES);
     o.Q = Q;
     o.Qp = round_up(Q, QB);
     o.qblocks = o.Qp / QB;
@@ -605,7 +602,8 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
         // GEMM-operand slices of B: rows q, columns (i, a) blocked by 64 along a.
         o.vlp = round_up(vl, 64);
         o.vrp = round_up(vr, 64);
-        const size_t gl = (size_t)S * Q * ol * o.vlp * 2, gr = Br ? (size_t)S * Q * orr * o.vrp * 2 : 0;
+        const int SG = o.gslices;
+        const size_t gl = (size_t)SG * Q * ol * o.vlp * 2, gr = Br ? (size_t)SG * Q * orr * o.vrp * 2 : 0;
         const size_t gel = (size_t)Q * ol * (o.vlp / 64), ger = Br ? (size_t)Q * orr * (o.vrp / 64) : 0;
         char* g = static_cast<char*>(workspace(ctx, WS_SLICE2, gl + gr + gel + ger + 1024));
         if (!g) return MSHQC_HIP_ERR_ALLOC;
@@ -613,11 +611,11 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
         o.gr = Br ? reinterpret_cast<uint16_t*>(g + gl) : o.gl;
         o.gel = reinterpret_cast<int8_t*>(g + gl + gr);
         o.ger = Br ? reinterpret_cast<int8_t*>(g + gl + gr + gel) : o.gel;
-        rc = by_slices<SliceRows>(S, ctx, Bl, (long)ovl, Q, ol, vl, o.vlp, const_cast<uint16_t*>(o.gl),
+        rc = by_slices<SliceRows>(SG, ctx, Bl, (long)ovl, Q, ol, vl, o.vlp, const_cast<uint16_t*>(o.gl),
                                   const_cast<int8_t*>(o.gel));
         if (rc) return rc;
         if (Br) {
-            rc = by_slices<SliceRows>(S, ctx, Br, (long)ovr, Q, orr, vr, o.vrp, const_cast<uint16_t*>(o.gr),
+            rc = by_slices<SliceRows>(SG, ctx, Br, (long)ovr, Q, orr, vr, o.vrp, const_cast<uint16_t*>(o.gr),
                                       const_cast<int8_t*>(o.ger));
             if (rc) return rc;
         }
@@ -637,7 +635,7 @@ int bf16_launch_pair(mshqc_hip_ctx* ctx, int mode, const PairArgs& a, const Bf16
 
 int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& o, const double* Z, long ldz, int I0,
                        int nI, int J0, int nJ, double* dBl, double* dBr, bool second) {
-    const int S = o.slices, Q = o.Q;
+    const int S = o.gslices, Q = o.Q;
     const int vl = o.vl, vr = o.vr, vlp = o.vlp, vrp = o.vrp;
     int rc;
     if (dBl) {

@@ -52,35 +52,38 @@ __global__ void occ_reduce_kernel(const int2* __restrict__ pairs, const double* 
 }
 
 // dE/deps_vir: row sums feed the left virtual index, column sums the right.
-__global__ void vir_reduce_kernel(const int2* __restrict__ tiles, int ntiles, int npairs,
-                                  const double* __restrict__ vrow, const double* __restrict__ vcol,
-                                  double* dev_l, int vl, double* dev_r, int vr, int sym) {
+// Only the tiles that touch the index's 64-block are visited: in the
+// symmetric modes the tile list holds (A,B), A <= B, in row-major order; in
+// the opposite-spin mode it holds all (A,B) with B fastest.
+__global__ void vir_reduce_kernel(int ntA, int ntB, int npairs, const double* __restrict__ vrow,
+                                  const double* __restrict__ vcol, double* dev_l, int vl,
+                                  double* dev_r, int vr, int sym) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     const int blk = a / TILE, off = a % TILE;
+    const int ntiles = sym ? ntA * (ntA + 1) / 2 : ntA * ntB;
+    auto sym_index = [ntA](int A, int B) { return A * ntA - A * (A - 1) / 2 + (B - A); };
     if (sym) {
         if (a >= vl) return;
         double acc = 0.0;
-        for (int q = 0; q < npairs; ++q)
-            for (int t = 0; t < ntiles; ++t) {
-                const int2 tb = tiles[t];
-                const long base = ((long)q * ntiles + t) * TILE + off;
-                if (tb.x == blk) acc += vrow[base];
-                if (tb.y == blk) acc += vcol[base];
-            }
+        for (int q = 0; q < npairs; ++q) {
+            const long base = (long)q * ntiles;
+            for (int B = blk; B < ntA; ++B) acc += vrow[(base + sym_index(blk, B)) * TILE + off];
+            for (int A = 0; A <= blk; ++A) acc += vcol[(base + sym_index(A, blk)) * TILE + off];
+        }
         dev_l[a] += acc;
     } else {
         if (a < vl && dev_l) {
             double acc = 0.0;
             for (int q = 0; q < npairs; ++q)
-                for (int t = 0; t < ntiles; ++t)
-                    if (tiles[t].x == blk) acc += vrow[((long)q * ntiles + t) * TILE + off];
+                for (int B = 0; B < ntB; ++B)
+                    acc += vrow[((long)q * ntiles + (long)blk * ntB + B) * TILE + off];
             dev_l[a] += acc;
         }
         if (a < vr && dev_r) {
             double acc = 0.0;
             for (int q = 0; q < npairs; ++q)
-                for (int t = 0; t < ntiles; ++t)
-                    if (tiles[t].y == blk) acc += vcol[((long)q * ntiles + t) * TILE + off];
+                for (int A = 0; A < ntA; ++A)
+                    acc += vcol[((long)q * ntiles + (long)A * ntB + blk) * TILE + off];
             dev_r[a] += acc;
         }
     }
@@ -268,7 +271,7 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
         if (pr.devl || pr.devr) {
             const int nv = std::max(pr.vl, pr.vr);
             vir_reduce_kernel<<<ceil_div(nv, 128), 128, 0, ctx->stream>>>(
-                d_tiles, ntiles, sg.n, a.vrow, a.vcol, pr.devl, pr.devl ? pr.vl : 0, pr.devr,
+                ntA, ntB, sg.n, a.vrow, a.vcol, pr.devl, pr.devl ? pr.vl : 0, pr.devr,
                 pr.devr ? pr.vr : 0, sym ? 1 : 0);
             LAUNCH_CHECK(ctx);
         }

@@ -2,15 +2,18 @@
 //
 // Each FP64 operand element x of a block (one row over one block of the
 // contraction index) is written as
-//     x = 2^e * sum_{p < S} m_p * 2^{-8(p+1)}  (+ truncation < 2^{e-8S-1})
+//     x = 2^{e-8} * sum_{p < S} s_p,   s_p = m_p * 2^{-8p}  (+ truncation < 2^{e-8S-1})
 // where 2^e bounds |x| over the block and every m_p is an integer with
-// |m_0| <= 256 and |m_p| <= 128 (p >= 1) -- exactly representable in BF16.
-// A slice product m_p * m_q is an exact integer below 2^16, so the products
-// of one class c = p + q, summed over a block of <= 128 contraction indices,
-// stay below 2^24 and are accumulated EXACTLY by the FP32 MFMA accumulator.
-// After every block the class accumulators are scaled by 2^{eA+eB-8(c+2)}
-// and added in FP64.  Products with p + q <= S-1 are kept (S(S+1)/2 MFMAs
-// per output tile instead of one FP64 MFMA).
+// |m_0| <= 256 and |m_p| <= 128 (p >= 1); each s_p is therefore exact in
+// BF16.  A slice product s_p s_q = m_p m_q 2^{-8(p+q)} is exact.  Products
+// are summed on the FP32 MFMA accumulator in three tiers by class c = p+q:
+//   tier 0 (c = 0): integers < 2^16, <= 128 per block -> exact
+//   tier 1 (c = 1): multiples of 2^-8 below 2^15     -> exact
+//   tier 2 (c >= 2): below 2^-1 per product; FP32 rounding there sits near
+//                    2^-40 of the tier-0 scale, under the truncation error.
+// After every block the tiers are added in FP64 and scaled by 2^{eA+eB-16}.
+// Products with p + q <= S-1 are kept (S(S+1)/2 MFMAs per output tile
+// instead of one FP64 MFMA).
 //
 // MFMA 16x16x32 bf16 lane layout (measured on gfx950 with one-hot operands):
 //   A: lane l holds A[row = l%16][k = 8*(l/16) + 0..7]
@@ -45,23 +48,28 @@ __device__ __forceinline__ double pow2(int k) {
     return __longlong_as_double((long long)(k + 1023) << 52);
 }
 
-// BF16 bit pattern of an integer-valued float with |m| <= 256 (exact).
+// BF16 bit pattern of a float carrying at most 8 significant bits (exact).
 __device__ __forceinline__ uint16_t bf16_bits_exact(float m) {
     return (uint16_t)(__float_as_uint(m) >> 16);
 }
 
-// Slice x (|x| < 2^e) into S BF16 integers (bit patterns in out[p]).
+// Slice x (|x| < 2^e) into S BF16 values s_p = m_p 2^{-8p} (bit patterns).
 template <int S>
 __device__ __forceinline__ void slice_value(double x, int e, uint16_t (&out)[S]) {
     double r = x * pow2(-e);  // |r| < 1 (exact scaling)
+    float sc = 1.0f;
 #pragma unroll
     for (int p = 0; p < S; ++p) {
         const double t = r * 256.0;
         const double m = rint(t);
-        out[p] = bf16_bits_exact((float)m);
+        out[p] = bf16_bits_exact((float)m * sc);
         r = t - m;  // exact: |r| <= 1/2
+        sc *= (1.0f / 256.0f);
     }
 }
+
+// Accumulator tier of a slice product p, q.
+__host__ __device__ constexpr int ozaki_tier(int p, int q) { return p + q < 2 ? p + q : 2; }
 
 // Number of kept products for S slices.
 __host__ __device__ constexpr int ozaki_products(int S) { return S * (S + 1) / 2; }

@@ -6,7 +6,8 @@ Bars (relative, stated per slice count):
     to the FP64 path's up to the last bits of the epilogue rounding
   * |dE|/|E| <= 1e-4 (k=2, the hi+lo three-product form), 1e-8 (k=3),
     1e-9 (k=4), 1e-12 (k=5), 1e-13 (k=6)
-  * gradient ||d dE/dB|| / ||dE/dB|| <= 1e-6 for k >= 3
+  * gradient ||d dE/dB|| / ||dE/dB|| <= 1e-6 for k >= 3 (the gradient
+    contractions use min(k, 3) slices)
 """
 import numpy as np
 import pytest
@@ -70,7 +71,7 @@ def test_grad_blocks_bf16(ctx):
     finally:
         c.close()
     rel = np.linalg.norm(to_np(g) - g_ref) / np.linalg.norm(g_ref)
-    assert rel <= 1e-9
+    assert rel <= 1e-6
     assert abs(E - E_ref) / abs(E_ref) <= 1e-12
 
 
@@ -86,4 +87,4 @@ def test_ump2_bf16(ctx):
         assert abs(x - y) <= 1e-12 * abs(y)
     for a, b in zip(grads[:2], grads_ref[:2]):
         a = to_np(a)
-        assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-9
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-6

@@ -1,0 +1,241 @@
+// Cholesky-vector drop-in classes: CholeskyERI, CholeskyRHF, CholeskyUHF,
+// CholeskyRMP2, CholeskyUMP2.  Public names, constructor arguments and
+// defaults follow syahrulhidayat/mshqc; the J/K build, the AO->MO transforms
+// and the (ia|jb)/denominator reductions run through libmshqc_hip.so.
+#pragma once
+
+#include <chrono>
+#include <memory>
+
+#include "core.h"
+#include "mshqc_hip.h"
+
+namespace mshqc {
+
+// ----------------------------------------------------------------- device
+
+namespace device {
+// Process-wide context on device 0 (created on first use).
+mshqc_hip_ctx* ctx();
+// Throws std::runtime_error with the library's message when rc != 0.
+void check(int rc, const char* what);
+
+// Owning device buffer of doubles.
+class Buffer {
+public:
+    Buffer() = default;
+    explicit Buffer(size_t count);
+    ~Buffer();
+    Buffer(const Buffer&) = delete;
+    Buffer& operator=(const Buffer&) = delete;
+    double* get() const { return p_; }
+    size_t count() const { return n_; }
+    void upload(const double* host, size_t count);
+    void download(double* host, size_t count) const;
+
+private:
+    double* p_ = nullptr;
+    size_t n_ = 0;
+};
+}  // namespace device
+
+// ----------------------------------------------------------- CholeskyERI
+
+struct CholeskyDecompositionResult {
+    int n_vectors = 0;
+    int n_basis = 0;
+    double threshold = 0.0;
+    double max_error = 0.0;
+    double rms_error = 0.0;
+    double compression_ratio = 0.0;
+    bool converged = false;
+};
+
+namespace integrals {
+
+// Pivoted incomplete Cholesky of the ERI matrix in FP64 on the host
+// (cholesky_eri.cc:40-203 semantics: diagonal over all n^2 ordered pairs
+// i*n+j, first maximum as pivot, stop when it falls below the threshold,
+// negative updated diagonals clamped to zero).  The vectors live on the host
+// and are uploaded to the device once, on first use by a Fock or MP2 call.
+class CholeskyERI {
+public:
+    explicit CholeskyERI(double threshold = 1e-6);
+    CholeskyDecompositionResult decompose(const ERITensor& eri);
+    double reconstruct(int i, int j, int a, int b) const;
+    int n_vectors() const { return (int)L_.size(); }
+    int n_basis() const { return n_basis_; }
+    double threshold() const { return threshold_; }
+    bool is_decomposed() const { return decomposed_; }
+    double compression_ratio() const;
+    size_t storage_bytes() const { return L_.size() * (size_t)n_basis_ * n_basis_ * sizeof(double); }
+    const std::vector<Vec>& get_L_vectors() const { return L_; }
+    void clear();
+    // Appends one vector of n*n values (index i*n+j) and counts it.
+    void add_vector(const Vec& v);
+    // Replaces all vectors (each n*n, index i*n+j).
+    void set_vectors(const std::vector<Vec>& vectors, int n_basis);
+    // Device copy [Q][n][n] (uploaded on first call).
+    const double* device_vectors() const;
+
+private:
+    std::vector<Vec> L_;
+    int n_basis_ = 0;
+    double threshold_ = 1e-6;
+    bool decomposed_ = false;
+    mutable std::shared_ptr<device::Buffer> dev_;
+};
+
+}  // namespace integrals
+
+// ------------------------------------------------------------- CD-RHF
+
+struct CholeskyRHFConfig : public SCFConfig {
+    double cholesky_threshold = 1e-6;
+    bool screen_exchange = true;
+    CholeskyRHFConfig() { print_level = 1; }
+};
+
+class CholeskyRHF {
+public:
+    CholeskyRHF(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                const CholeskyRHFConfig& config);
+    CholeskyRHF(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                const CholeskyRHFConfig& config, const integrals::CholeskyERI& existing_cholesky);
+    SCFResult compute();
+    double energy() const { return energy_; }
+
+private:
+    Molecule mol_;
+    BasisSet basis_;
+    std::shared_ptr<IntegralEngine> integrals_;
+    CholeskyRHFConfig config_;
+    std::unique_ptr<integrals::CholeskyERI> internal_;
+    const integrals::CholeskyERI* chol_ = nullptr;
+    DIIS diis_;
+    int nbasis_ = 0, n_occ_ = 0;
+    Mat S_, H_, X_, F_, P_, C_;
+    Vec eps_;
+    double energy_ = 0.0;
+
+    void init_integrals();
+    void initial_guess();
+    Mat build_density() const;
+    void build_fock();
+    void solve_fock();
+};
+
+// ------------------------------------------------------------- CD-UHF
+
+struct CholeskyUHFConfig {
+    double energy_threshold = 1e-8;
+    double cholesky_threshold = 1e-6;
+    int max_iterations = 100;
+    int diis_max_vectors = 6;
+    int print_level = 1;
+};
+
+class CholeskyUHF {
+public:
+    CholeskyUHF(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                int n_alpha, int n_beta, const CholeskyUHFConfig& config);
+    SCFResult compute();
+    void set_cholesky_vectors(const std::vector<Vec>& vectors);
+    double compute_s_squared() const;
+
+private:
+    Molecule mol_;
+    BasisSet basis_;
+    std::shared_ptr<IntegralEngine> integrals_;
+    CholeskyUHFConfig config_;
+    int n_alpha_, n_beta_, nbasis_;
+    integrals::CholeskyERI chol_;
+    bool external_ = false;
+    Mat S_, H_, X_, Fa_, Fb_, Pa_, Pb_, Ca_, Cb_;
+    Vec eps_a_, eps_b_;
+    double energy_ = 0.0, energy_old_ = 0.0;
+    DIIS diis_a_, diis_b_;
+
+    void init_integrals_and_cholesky();
+    void initial_guess();
+    void build_fock_cholesky();
+    void solve_fock(const Mat& F, Mat& C, Vec& eps) const;
+    Mat build_density(const Mat& C, int n_occ) const;
+    double compute_energy() const;
+};
+
+// ------------------------------------------------------------- CD-RMP2
+
+namespace foundation {
+struct CholeskyRMP2Result {
+    double e_rhf = 0.0, e_corr = 0.0, e_total = 0.0;
+    // t2[i][j][a][b] (flattened, o*o*v*v) when stored; empty otherwise.
+    Vec t2;
+    int n_chol_vectors = 0;
+    int n_occ = 0, n_virt = 0;
+    double time_transform_s = 0.0, time_energy_s = 0.0;
+};
+}  // namespace foundation
+
+struct CholeskyRMP2Config {
+    double cholesky_threshold = 1e-6;
+    int print_level = 1;
+    // 0 = FP64; k >= 2 = split-BF16 with k slices (see mshqc_hip.h).
+    int precision = 0;
+    // Amplitudes kept for later use (the reference always stores them):
+    // 1 = always, 0 = never, -1 = when o^2 v^2 <= 2^27.
+    int store_t2 = -1;
+};
+
+class CholeskyRMP2 {
+public:
+    CholeskyRMP2(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                 const SCFResult& rhf_result, const CholeskyRMP2Config& config);
+    CholeskyRMP2(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                 const SCFResult& rhf_result, const CholeskyRMP2Config& config,
+                 const integrals::CholeskyERI& existing_cholesky);
+    foundation::CholeskyRMP2Result compute();
+
+private:
+    SCFResult rhf_;
+    CholeskyRMP2Config config_;
+    std::unique_ptr<integrals::CholeskyERI> internal_;
+    const integrals::CholeskyERI* chol_ = nullptr;
+    int nbf_, nocc_, nvirt_;
+};
+
+// ------------------------------------------------------------- CD-UMP2
+
+struct CholeskyUMP2Config {
+    double cholesky_threshold = 1e-6;
+    bool use_on_the_fly = false;
+    bool validate_energy = false;
+    int print_level = 1;
+    int precision = 0;
+};
+
+struct CholeskyUMP2Result {
+    double e_corr_ss_aa = 0.0, e_corr_ss_bb = 0.0, e_corr_os = 0.0, e_corr_total = 0.0, e_total = 0.0;
+    int n_cholesky_vectors = 0;
+    double compression_ratio = 0.0, memory_mb = 0.0;
+    double time_cholesky_s = 0.0, time_transform_s = 0.0, time_energy_s = 0.0;
+};
+
+class CholeskyUMP2 {
+public:
+    CholeskyUMP2(const SCFResult& uhf_result, const BasisSet& basis,
+                 std::shared_ptr<IntegralEngine> integrals, const CholeskyUMP2Config& config,
+                 const integrals::CholeskyERI& cholesky_from_uhf);
+    CholeskyUMP2Result compute();
+    const integrals::CholeskyERI& get_cholesky() const { return chol_; }
+
+private:
+    SCFResult uhf_;
+    BasisSet basis_;
+    std::shared_ptr<IntegralEngine> integrals_;
+    CholeskyUMP2Config config_;
+    integrals::CholeskyERI chol_;  // shares the device copy with the source
+    int nbf_, na_, nb_, va_, vb_;
+};
+
+}  // namespace mshqc

@@ -1,0 +1,686 @@
+// Host data layer: dense linear algebra, Molecule, BasisSet, the
+// McMurchie-Davidson integral engine and DIIS.
+#include "core.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <fstream>
+#include <sstream>
+
+namespace mshqc {
+
+// ================================================================ algebra
+
+Mat Mat::identity(int n) {
+    Mat I(n, n);
+    for (int i = 0; i < n; ++i) I(i, i) = 1.0;
+    return I;
+}
+
+Mat matmul(const Mat& A, const Mat& B) {
+    if (A.cols != B.rows) throw std::invalid_argument("matmul: shape mismatch");
+    Mat C(A.rows, B.cols);
+#pragma omp parallel for if ((long)A.rows * B.cols * A.cols > 1000000)
+    for (int i = 0; i < A.rows; ++i)
+        for (int k = 0; k < A.cols; ++k) {
+            const double a = A(i, k);
+            const double* b = &B.a[(size_t)k * B.cols];
+            double* c = &C.a[(size_t)i * C.cols];
+            for (int j = 0; j < B.cols; ++j) c[j] += a * b[j];
+        }
+    return C;
+}
+
+Mat transpose(const Mat& A) {
+    Mat T(A.cols, A.rows);
+    for (int i = 0; i < A.rows; ++i)
+        for (int j = 0; j < A.cols; ++j) T(j, i) = A(i, j);
+    return T;
+}
+
+Mat matmul_tn(const Mat& A, const Mat& B) { return matmul(transpose(A), B); }
+Mat matmul_nt(const Mat& A, const Mat& B) { return matmul(A, transpose(B)); }
+
+Mat add(const Mat& A, const Mat& B, double beta) {
+    if (A.rows != B.rows || A.cols != B.cols) throw std::invalid_argument("add: shape mismatch");
+    Mat C = A;
+    for (size_t k = 0; k < C.a.size(); ++k) C.a[k] += beta * B.a[k];
+    return C;
+}
+
+Mat scale(const Mat& A, double s) {
+    Mat C = A;
+    for (auto& x : C.a) x *= s;
+    return C;
+}
+
+double trace_product(const Mat& A, const Mat& B) {
+    double t = 0.0;
+    for (int i = 0; i < A.rows; ++i)
+        for (int k = 0; k < A.cols; ++k) t += A(i, k) * B(k, i);
+    return t;
+}
+
+double dot(const Mat& A, const Mat& B) {
+    double t = 0.0;
+    for (size_t k = 0; k < A.a.size(); ++k) t += A.a[k] * B.a[k];
+    return t;
+}
+
+Mat left_cols(const Mat& A, int k) {
+    Mat R(A.rows, k);
+    for (int i = 0; i < A.rows; ++i)
+        for (int j = 0; j < k; ++j) R(i, j) = A(i, j);
+    return R;
+}
+
+Mat right_cols(const Mat& A, int k) {
+    Mat R(A.rows, k);
+    const int off = A.cols - k;
+    for (int i = 0; i < A.rows; ++i)
+        for (int j = 0; j < k; ++j) R(i, j) = A(i, off + j);
+    return R;
+}
+
+void eigh(const Mat& Ain, Vec& w, Mat& V) {
+    const int n = Ain.rows;
+    if (Ain.cols != n) throw std::invalid_argument("eigh: matrix must be square");
+    Mat A = Ain;
+    // symmetrise against round-off in the input
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) A(i, j) = A(j, i) = 0.5 * (A(i, j) + A(j, i));
+    V = Mat::identity(n);
+    double fro = 0.0;
+    for (double x : A.a) fro += x * x;
+    fro = std::sqrt(fro);
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) off += A(p, q) * A(p, q);
+        if (std::sqrt(off) <= 1e-15 * fro || off == 0.0) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A(p, q);
+                if (std::fabs(apq) <= 1e-300) continue;
+                // Rotation in the (p,q) plane that annihilates A(p,q).
+                const double tau = (A(q, q) - A(p, p)) / (2.0 * apq);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int r = 0; r < n; ++r) {
+                    const double arp = A(r, p), arq = A(r, q);
+                    A(r, p) = c * arp - s * arq;
+                    A(r, q) = s * arp + c * arq;
+                }
+                for (int r = 0; r < n; ++r) {
+                    const double apr = A(p, r), aqr = A(q, r);
+                    A(p, r) = c * apr - s * aqr;
+                    A(q, r) = s * apr + c * aqr;
+                }
+                for (int r = 0; r < n; ++r) {
+                    const double vrp = V(r, p), vrq = V(r, q);
+                    V(r, p) = c * vrp - s * vrq;
+                    V(r, q) = s * vrp + c * vrq;
+                }
+            }
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return A(a, a) < A(b, b); });
+    w.assign(n, 0.0);
+    Mat Vs(n, n);
+    for (int k = 0; k < n; ++k) {
+        w[k] = A(idx[k], idx[k]);
+        for (int r = 0; r < n; ++r) Vs(r, k) = V(r, idx[k]);
+    }
+    V = Vs;
+}
+
+Vec solve(Mat A, Vec b) {
+    const int n = A.rows;
+    std::vector<int> colperm(n);
+    for (int i = 0; i < n; ++i) colperm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        int pr = k, pc = k;
+        double best = -1.0;
+        for (int i = k; i < n; ++i)
+            for (int j = k; j < n; ++j)
+                if (std::fabs(A(i, j)) > best) best = std::fabs(A(i, j)), pr = i, pc = j;
+        if (best <= 0.0) continue;  // singular block: leave as is
+        if (pr != k) {
+            for (int j = 0; j < n; ++j) std::swap(A(k, j), A(pr, j));
+            std::swap(b[k], b[pr]);
+        }
+        if (pc != k) {
+            for (int i = 0; i < n; ++i) std::swap(A(i, k), A(i, pc));
+            std::swap(colperm[k], colperm[pc]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            const double f = A(i, k) / A(k, k);
+            if (f == 0.0) continue;
+            for (int j = k; j < n; ++j) A(i, j) -= f * A(k, j);
+            b[i] -= f * b[k];
+        }
+    }
+    Vec y(n, 0.0);
+    for (int i = n - 1; i >= 0; --i) {
+        double s = b[i];
+        for (int j = i + 1; j < n; ++j) s -= A(i, j) * y[j];
+        y[i] = A(i, i) != 0.0 ? s / A(i, i) : 0.0;
+    }
+    Vec x(n, 0.0);
+    for (int i = 0; i < n; ++i) x[colperm[i]] = y[i];
+    return x;
+}
+
+// ================================================================ molecule
+
+const Atom& Molecule::atom(int i) const {
+    if (i < 0 || i >= n_atoms()) throw std::out_of_range("Molecule::atom: index out of range");
+    return atoms_[i];
+}
+
+int Molecule::total_nuclear_charge() const {
+    int z = 0;
+    for (const auto& a : atoms_) z += a.atomic_number;
+    return z;
+}
+
+double Molecule::nuclear_repulsion_energy() const {
+    double e = 0.0;
+    for (size_t i = 0; i < atoms_.size(); ++i)
+        for (size_t j = i + 1; j < atoms_.size(); ++j) {
+            const double dx = atoms_[i].x - atoms_[j].x, dy = atoms_[i].y - atoms_[j].y,
+                         dz = atoms_[i].z - atoms_[j].z;
+            e += atoms_[i].atomic_number * atoms_[j].atomic_number / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    return e;
+}
+
+// =================================================================== basis
+
+namespace {
+
+const char* kSymbols[] = {"",   "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",
+                          "Ne", "Na", "Mg", "Al", "Si", "P",  "S",  "Cl", "Ar", "K",
+                          "Ca", "Sc", "Ti", "V",  "Cr", "Mn", "Fe", "Co", "Ni", "Cu",
+                          "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr"};
+
+double double_factorial(int n) {
+    double r = 1.0;
+    for (int k = n; k > 1; k -= 2) r *= k;
+    return r;
+}
+
+std::string upper(std::string s) {
+    for (auto& c : s) c = (char)std::toupper((unsigned char)c);
+    return s;
+}
+
+double parse_number(std::string s) {
+    for (auto& c : s)
+        if (c == 'D' || c == 'd') c = 'E';
+    return std::stod(s);
+}
+
+int am_of(char c) {
+    const std::string L = "SPDFGH";
+    const auto k = L.find((char)std::toupper((unsigned char)c));
+    return k == std::string::npos ? -1 : (int)k;
+}
+
+}  // namespace
+
+int Shell::n_functions() const {
+    const int L = l();
+    return spherical_ ? 2 * L + 1 : (L + 1) * (L + 2) / 2;
+}
+
+void Shell::normalize() {
+    const int L = l();
+    for (auto& p : primitives_) {
+        const double N = std::pow(2.0 * p.exponent / M_PI, 0.75) * std::pow(4.0 * p.exponent, L / 2.0) /
+                         std::sqrt(double_factorial(2 * L - 1));
+        p.coefficient *= N;
+    }
+    double sum = 0.0;
+    for (const auto& a : primitives_)
+        for (const auto& b : primitives_) {
+            const double ratio = 2.0 * std::sqrt(a.exponent * b.exponent) / (a.exponent + b.exponent);
+            sum += a.coefficient * b.coefficient * std::pow(ratio, L + 1.5);
+        }
+    if (sum > 1e-14) {
+        const double k = 1.0 / std::sqrt(sum);
+        for (auto& p : primitives_) p.coefficient *= k;
+    }
+}
+
+BasisSet::BasisSet(const std::string& basis_name, const Molecule& mol, const std::string& basis_dir)
+    : name_(basis_name) {
+    std::string lower = basis_name;
+    for (auto& c : lower) c = (char)std::tolower((unsigned char)c);
+    const std::string file = basis_dir + "/" + lower + ".gbs";
+    if (!read_gbs(file, mol)) throw std::runtime_error("Failed to read basis set: " + file);
+}
+
+bool BasisSet::read_gbs(const std::string& file, const Molecule& mol) {
+    std::ifstream f(file);
+    if (!f) return false;
+    std::vector<std::string> lines;
+    std::string line;
+    while (std::getline(f, line)) lines.push_back(line);
+    if (lines.empty()) return false;
+    spherical_ = lines[0].find("cartesian") == std::string::npos;
+    shells_.clear();
+    for (int ia = 0; ia < mol.n_atoms(); ++ia) {
+        const Atom& at = mol.atom(ia);
+        if (at.atomic_number <= 0 || at.atomic_number > 36)
+            throw std::runtime_error("read_gbs: unsupported element Z=" + std::to_string(at.atomic_number));
+        const std::string sym = upper(kSymbols[at.atomic_number]);
+        // Locate "SYM 0" right after a "****" line.
+        size_t k = 0;
+        bool found = false;
+        for (; k + 1 < lines.size(); ++k) {
+            if (lines[k].find("****") == std::string::npos) continue;
+            std::istringstream is(lines[k + 1]);
+            std::string s;
+            if ((is >> s) && upper(s) == sym) {
+                found = true;
+                k += 2;
+                break;
+            }
+        }
+        if (!found) throw std::runtime_error("read_gbs: element " + sym + " not in " + file);
+        const std::array<double, 3> pos = at.position();
+        while (k < lines.size() && lines[k].find("****") == std::string::npos) {
+            std::istringstream is(lines[k]);
+            std::string type;
+            int nprim = 0;
+            if (!(is >> type >> nprim)) {
+                ++k;
+                continue;
+            }
+            type = upper(type);
+            ++k;
+            if (type == "SP") {
+                Shell s(AngularMomentum::S, ia, pos), p(AngularMomentum::P, ia, pos);
+                s.set_spherical(spherical_);
+                p.set_spherical(spherical_);
+                for (int i = 0; i < nprim && k < lines.size(); ++i, ++k) {
+                    std::istringstream ps(lines[k]);
+                    std::string e, cs, cp;
+                    ps >> e >> cs >> cp;
+                    s.add_primitive(parse_number(e), parse_number(cs));
+                    p.add_primitive(parse_number(e), parse_number(cp));
+                }
+                shells_.push_back(s);
+                shells_.push_back(p);
+            } else {
+                const int L = am_of(type[0]);
+                if (L < 0 || type.size() != 1) throw std::runtime_error("read_gbs: bad shell type " + type);
+                Shell s((AngularMomentum)L, ia, pos);
+                s.set_spherical(spherical_);
+                for (int i = 0; i < nprim && k < lines.size(); ++i, ++k) {
+                    std::istringstream ps(lines[k]);
+                    std::string e, c;
+                    ps >> e >> c;
+                    s.add_primitive(parse_number(e), parse_number(c));
+                }
+                shells_.push_back(s);
+            }
+        }
+    }
+    for (auto& s : shells_) s.normalize();
+    return true;
+}
+
+int BasisSet::n_basis_functions() const {
+    int n = 0;
+    for (const auto& s : shells_) n += s.n_functions();
+    return n;
+}
+
+const Shell& BasisSet::shell(int i) const {
+    if (i < 0 || i >= n_shells()) throw std::out_of_range("BasisSet::shell: index out of range");
+    return shells_[i];
+}
+
+// ========================================================= integral engine
+
+namespace {
+
+constexpr int LMAX = 1;                 // per shell (s, p)
+constexpr int TMAX = 4 * LMAX;          // Hermite index bound in an ERI
+
+// Cartesian exponents of component c of a shell with angular momentum L
+// (order x, y, z for L = 1).
+void cart(int L, int c, int& lx, int& ly, int& lz) {
+    int idx = 0;
+    for (int a = L; a >= 0; --a)
+        for (int b = L - a; b >= 0; --b) {
+            if (idx == c) {
+                lx = a;
+                ly = b;
+                lz = L - a - b;
+                return;
+            }
+            ++idx;
+        }
+}
+
+// Boys function F_n(T), n = 0..nmax: series for the top order, downward
+// recursion below (F_n = (2T F_{n+1} + e^-T) / (2n+1)); asymptotic form for
+// large T.
+void boys(int nmax, double T, double* F) {
+    if (T < 35.0) {
+        const double e = std::exp(-T);
+        double term = 1.0 / (2 * nmax + 1), sum = term;
+        for (int k = 1; k < 400; ++k) {
+            term *= 2.0 * T / (2 * nmax + 2 * k + 1);
+            sum += term;
+            if (term < 1e-17 * sum) break;
+        }
+        F[nmax] = e * sum;
+        for (int n = nmax - 1; n >= 0; --n) F[n] = (2.0 * T * F[n + 1] + e) / (2 * n + 1);
+    } else {
+        F[0] = 0.5 * std::sqrt(M_PI / T);
+        for (int n = 1; n <= nmax; ++n) F[n] = F[n - 1] * (2 * n - 1) / (2.0 * T);
+    }
+}
+
+// 1-D Hermite expansion coefficients E[i][j][t] for exponents a, b, centres
+// separated by Xab = A - B.
+struct Herm {
+    double E[LMAX + 3][LMAX + 3][2 * LMAX + 5];
+};
+
+void hermite(double a, double b, double Xab, int imax, int jmax, Herm& h) {
+    const double p = a + b, mu = a * b / p;
+    const double Xpa = -b * Xab / p, Xpb = a * Xab / p;
+    for (int i = 0; i <= imax; ++i)
+        for (int j = 0; j <= jmax; ++j)
+            for (int t = 0; t <= i + j + 1; ++t) h.E[i][j][t] = 0.0;
+    h.E[0][0][0] = std::exp(-mu * Xab * Xab);
+    for (int i = 0; i <= imax; ++i)
+        for (int j = 0; j <= jmax; ++j) {
+            if (i == 0 && j == 0) continue;
+            for (int t = 0; t <= i + j; ++t) {
+                double v = 0.0;
+                if (i > 0) {
+                    if (t > 0) v += h.E[i - 1][j][t - 1] / (2 * p);
+                    v += Xpa * h.E[i - 1][j][t];
+                    if (t + 1 <= i - 1 + j) v += (t + 1) * h.E[i - 1][j][t + 1];
+                } else {
+                    if (t > 0) v += h.E[i][j - 1][t - 1] / (2 * p);
+                    v += Xpb * h.E[i][j - 1][t];
+                    if (t + 1 <= i + j - 1) v += (t + 1) * h.E[i][j - 1][t + 1];
+                }
+                h.E[i][j][t] = v;
+            }
+        }
+}
+
+// Hermite Coulomb integrals R_{tuv}^{(0)}(p, PC) for t+u+v <= nmax.
+struct RTab {
+    double R[TMAX + 1][TMAX + 1][TMAX + 1];
+};
+
+void hermite_coulomb(int nmax, double p, double X, double Y, double Z, RTab& out) {
+    static thread_local double Rn[TMAX + 1][TMAX + 1][TMAX + 1][TMAX + 1];
+    double F[TMAX + 1];
+    boys(nmax, p * (X * X + Y * Y + Z * Z), F);
+    double pw = 1.0;
+    for (int n = 0; n <= nmax; ++n) {
+        Rn[n][0][0][0] = pw * F[n];
+        pw *= -2.0 * p;
+    }
+    // Build R^n_{tuv} for t+u+v <= nmax - n, top-down in n.
+    for (int n = nmax - 1; n >= 0; --n) {
+        const int lim = nmax - n;
+        for (int t = 0; t <= lim; ++t)
+            for (int u = 0; u <= lim - t; ++u)
+                for (int v = 0; v <= lim - t - u; ++v) {
+                    if (t + u + v == 0) continue;
+                    double r;
+                    if (t > 0) {
+                        r = X * Rn[n + 1][t - 1][u][v];
+                        if (t > 1) r += (t - 1) * Rn[n + 1][t - 2][u][v];
+                    } else if (u > 0) {
+                        r = Y * Rn[n + 1][t][u - 1][v];
+                        if (u > 1) r += (u - 1) * Rn[n + 1][t][u - 2][v];
+                    } else {
+                        r = Z * Rn[n + 1][t][u][v - 1];
+                        if (v > 1) r += (v - 1) * Rn[n + 1][t][u][v - 2];
+                    }
+                    Rn[n][t][u][v] = r;
+                }
+    }
+    for (int t = 0; t <= nmax; ++t)
+        for (int u = 0; u <= nmax - t; ++u)
+            for (int v = 0; v <= nmax - t - u; ++v) out.R[t][u][v] = Rn[0][t][u][v];
+}
+
+void check_shells(const BasisSet& b) {
+    for (const auto& s : b.shells())
+        if (s.l() > LMAX)
+            throw std::runtime_error(
+                "IntegralEngine: shells above L=1 are not supported by this build's integral "
+                "engine (the reference uses libint2, which is not available here)");
+}
+
+}  // namespace
+
+IntegralEngine::IntegralEngine(const Molecule& mol, const BasisSet& basis) : mol_(mol), basis_(basis) {
+    check_shells(basis_);
+    nbf_ = 0;
+    for (const auto& s : basis_.shells()) {
+        offset_.push_back(nbf_);
+        nbf_ += s.n_functions();
+    }
+}
+
+namespace {
+enum OneKind { OVERLAP, KINETIC, NUCLEAR };
+
+Mat one_body(const BasisSet& basis, const std::vector<int>& offset, int nbf, const Molecule& mol,
+             OneKind kind) {
+    Mat M(nbf, nbf);
+    const auto& sh = basis.shells();
+    const int ns = (int)sh.size();
+    for (int s1 = 0; s1 < ns; ++s1)
+        for (int s2 = 0; s2 <= s1; ++s2) {
+            const Shell &A = sh[s1], &B = sh[s2];
+            const int LA = A.l(), LB = B.l();
+            const auto &PA = A.position(), &PB = B.position();
+            for (const auto& pa : A.primitives())
+                for (const auto& pb : B.primitives()) {
+                    const double a = pa.exponent, b = pb.exponent, p = a + b;
+                    Herm hx, hy, hz;
+                    hermite(a, b, PA[0] - PB[0], LA, LB + 2, hx);
+                    hermite(a, b, PA[1] - PB[1], LA, LB + 2, hy);
+                    hermite(a, b, PA[2] - PB[2], LA, LB + 2, hz);
+                    const double cc = pa.coefficient * pb.coefficient;
+                    const double s3 = std::pow(M_PI / p, 1.5);
+                    RTab R;
+                    for (int ca = 0; ca < A.n_functions(); ++ca)
+                        for (int cb = 0; cb < B.n_functions(); ++cb) {
+                            int ax, ay, az, bx, by, bz;
+                            cart(LA, ca, ax, ay, az);
+                            cart(LB, cb, bx, by, bz);
+                            double val = 0.0;
+                            if (kind == OVERLAP) {
+                                val = hx.E[ax][bx][0] * hy.E[ay][by][0] * hz.E[az][bz][0] * s3;
+                            } else if (kind == KINETIC) {
+                                auto k1 = [&](const Herm& h, int i, int j) {
+                                    double t = b * (2 * j + 1) * h.E[i][j][0] - 2 * b * b * h.E[i][j + 2][0];
+                                    if (j >= 2) t -= 0.5 * j * (j - 1) * h.E[i][j - 2][0];
+                                    return t;
+                                };
+                                val = (k1(hx, ax, bx) * hy.E[ay][by][0] * hz.E[az][bz][0] +
+                                       hx.E[ax][bx][0] * k1(hy, ay, by) * hz.E[az][bz][0] +
+                                       hx.E[ax][bx][0] * hy.E[ay][by][0] * k1(hz, az, bz)) * s3;
+                            } else {
+                                const double Px = (a * PA[0] + b * PB[0]) / p, Py = (a * PA[1] + b * PB[1]) / p,
+                                             Pz = (a * PA[2] + b * PB[2]) / p;
+                                for (const auto& at : mol.atoms()) {
+                                    const int nmax = ax + bx + ay + by + az + bz;
+                                    hermite_coulomb(nmax, p, Px - at.x, Py - at.y, Pz - at.z, R);
+                                    double s = 0.0;
+                                    for (int t = 0; t <= ax + bx; ++t)
+                                        for (int u = 0; u <= ay + by; ++u)
+                                            for (int v = 0; v <= az + bz; ++v)
+                                                s += hx.E[ax][bx][t] * hy.E[ay][by][u] * hz.E[az][bz][v] *
+                                                     R.R[t][u][v];
+                                    val -= at.atomic_number * 2.0 * M_PI / p * s;
+                                }
+                            }
+                            M(offset[s1] + ca, offset[s2] + cb) += cc * val;
+                        }
+                }
+            if (s1 != s2)
+                for (int ca = 0; ca < A.n_functions(); ++ca)
+                    for (int cb = 0; cb < B.n_functions(); ++cb)
+                        M(offset[s2] + cb, offset[s1] + ca) = M(offset[s1] + ca, offset[s2] + cb);
+        }
+    return M;
+}
+}  // namespace
+
+Mat IntegralEngine::compute_overlap() const { return one_body(basis_, offset_, nbf_, mol_, OVERLAP); }
+Mat IntegralEngine::compute_kinetic() const { return one_body(basis_, offset_, nbf_, mol_, KINETIC); }
+Mat IntegralEngine::compute_nuclear() const { return one_body(basis_, offset_, nbf_, mol_, NUCLEAR); }
+
+ERITensor IntegralEngine::compute_eri() const {
+    ERITensor eri(nbf_);
+    const auto& sh = basis_.shells();
+    const int ns = (int)sh.size();
+    // Shell pairs (s1 >= s2), then quartets with pair1 >= pair2 (8-fold symmetry).
+    std::vector<std::pair<int, int>> pairs;
+    for (int s1 = 0; s1 < ns; ++s1)
+        for (int s2 = 0; s2 <= s1; ++s2) pairs.emplace_back(s1, s2);
+    const long np = (long)pairs.size();
+#pragma omp parallel for schedule(dynamic)
+    for (long p1 = 0; p1 < np; ++p1) {
+        for (long p2 = 0; p2 <= p1; ++p2) {
+            const int sa = pairs[p1].first, sb = pairs[p1].second, sc = pairs[p2].first,
+                      sd = pairs[p2].second;
+            const Shell &A = sh[sa], &B = sh[sb], &C = sh[sc], &D = sh[sd];
+            const int na = A.n_functions(), nb = B.n_functions(), nc = C.n_functions(),
+                      nd = D.n_functions();
+            double buf[3][3][3][3] = {};
+            const auto &RA = A.position(), &RB = B.position(), &RC = C.position(), &RD = D.position();
+            for (const auto& pa : A.primitives())
+                for (const auto& pb : B.primitives()) {
+                    const double a = pa.exponent, b = pb.exponent, p = a + b;
+                    Herm ex, ey, ez;
+                    hermite(a, b, RA[0] - RB[0], A.l(), B.l(), ex);
+                    hermite(a, b, RA[1] - RB[1], A.l(), B.l(), ey);
+                    hermite(a, b, RA[2] - RB[2], A.l(), B.l(), ez);
+                    const double P[3] = {(a * RA[0] + b * RB[0]) / p, (a * RA[1] + b * RB[1]) / p,
+                                         (a * RA[2] + b * RB[2]) / p};
+                    for (const auto& pc : C.primitives())
+                        for (const auto& pd : D.primitives()) {
+                            const double c = pc.exponent, d = pd.exponent, q = c + d;
+                            Herm fx, fy, fz;
+                            hermite(c, d, RC[0] - RD[0], C.l(), D.l(), fx);
+                            hermite(c, d, RC[1] - RD[1], C.l(), D.l(), fy);
+                            hermite(c, d, RC[2] - RD[2], C.l(), D.l(), fz);
+                            const double Qc[3] = {(c * RC[0] + d * RD[0]) / q, (c * RC[1] + d * RD[1]) / q,
+                                                  (c * RC[2] + d * RD[2]) / q};
+                            const double alpha = p * q / (p + q);
+                            const double pref = 2.0 * std::pow(M_PI, 2.5) / (p * q * std::sqrt(p + q)) *
+                                                pa.coefficient * pb.coefficient * pc.coefficient *
+                                                pd.coefficient;
+                            const int nmax = A.l() + B.l() + C.l() + D.l();
+                            RTab R;
+                            hermite_coulomb(nmax, alpha, P[0] - Qc[0], P[1] - Qc[1], P[2] - Qc[2], R);
+                            for (int ia = 0; ia < na; ++ia)
+                                for (int ib = 0; ib < nb; ++ib)
+                                    for (int ic = 0; ic < nc; ++ic)
+                                        for (int id = 0; id < nd; ++id) {
+                                            int ax, ay, az, bx, by, bz, cx, cy, cz, dx, dy, dz;
+                                            cart(A.l(), ia, ax, ay, az);
+                                            cart(B.l(), ib, bx, by, bz);
+                                            cart(C.l(), ic, cx, cy, cz);
+                                            cart(D.l(), id, dx, dy, dz);
+                                            double s = 0.0;
+                                            for (int t = 0; t <= ax + bx; ++t)
+                                                for (int u = 0; u <= ay + by; ++u)
+                                                    for (int v = 0; v <= az + bz; ++v) {
+                                                        const double e1 = ex.E[ax][bx][t] * ey.E[ay][by][u] *
+                                                                          ez.E[az][bz][v];
+                                                        if (e1 == 0.0) continue;
+                                                        for (int tt = 0; tt <= cx + dx; ++tt)
+                                                            for (int uu = 0; uu <= cy + dy; ++uu)
+                                                                for (int vv = 0; vv <= cz + dz; ++vv) {
+                                                                    const double e2 = fx.E[cx][dx][tt] *
+                                                                                      fy.E[cy][dy][uu] *
+                                                                                      fz.E[cz][dz][vv];
+                                                                    const double sg =
+                                                                        ((tt + uu + vv) & 1) ? -1.0 : 1.0;
+                                                                    s += e1 * sg * e2 *
+                                                                         R.R[t + tt][u + uu][v + vv];
+                                                                }
+                                                    }
+                                            buf[ia][ib][ic][id] += pref * s;
+                                        }
+                        }
+                }
+            for (int ia = 0; ia < na; ++ia)
+                for (int ib = 0; ib < nb; ++ib)
+                    for (int ic = 0; ic < nc; ++ic)
+                        for (int id = 0; id < nd; ++id) {
+                            const int i = offset_[sa] + ia, j = offset_[sb] + ib, k = offset_[sc] + ic,
+                                      l = offset_[sd] + id;
+                            const double v = buf[ia][ib][ic][id];
+                            eri(i, j, k, l) = v;
+                            eri(j, i, k, l) = v;
+                            eri(i, j, l, k) = v;
+                            eri(j, i, l, k) = v;
+                            eri(k, l, i, j) = v;
+                            eri(l, k, i, j) = v;
+                            eri(k, l, j, i) = v;
+                            eri(l, k, j, i) = v;
+                        }
+        }
+    }
+    return eri;
+}
+
+// ==================================================================== DIIS
+
+DIIS::DIIS(int max_vectors) : max_vectors_(max_vectors) {
+    if (max_vectors < 2) throw std::invalid_argument("DIIS requires at least 2 vectors");
+}
+
+void DIIS::add_iteration(const Mat& F, const Mat& error) {
+    focks_.push_back(F);
+    errors_.push_back(error);
+    if ((int)focks_.size() > max_vectors_) {
+        focks_.erase(focks_.begin());
+        errors_.erase(errors_.begin());
+    }
+}
+
+Mat DIIS::extrapolate() const {
+    if (!can_extrapolate()) throw std::runtime_error("DIIS: not enough vectors");
+    const int n = (int)focks_.size();
+    Mat A(n + 1, n + 1);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) A(i, j) = dot(errors_[i], errors_[j]);
+    for (int i = 0; i < n; ++i) A(i, n) = A(n, i) = -1.0;
+    Vec b(n + 1, 0.0);
+    b[n] = -1.0;
+    const Vec x = solve(A, b);
+    Mat F(focks_[0].rows, focks_[0].cols);
+    for (int i = 0; i < n; ++i)
+        for (size_t k = 0; k < F.a.size(); ++k) F.a[k] += x[i] * focks_[i].a[k];
+    return F;
+}
+
+void DIIS::clear() {
+    focks_.clear();
+    errors_.clear();
+}
+
+}  // namespace mshqc

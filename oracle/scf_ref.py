@@ -1,0 +1,116 @@
+"""numpy restatement of the CD-RHF / CD-UHF drivers -- TEST INFRASTRUCTURE ONLY.
+
+Used by tests to check the drop-in SCF classes, whose J/K go through the HIP
+library.  Control flow follows the reference drivers step by step:
+
+* ``cd_rhf``: src/scf/cholesky_rhf.cc:157-254 -- core-Hamiltonian guess
+  (:80-88), F = H + J - K/2 (:99-137, with screen_exchange), the DIIS block
+  that never engages (:188-196), 0.7/0.3 density damping (:206-210),
+  E = 1/2 tr P (H + F) + E_nuc (:142-144), stop on |dE| < thr after iter 1.
+* ``cd_uhf``: src/scf/cholesky_uhf.cc:121-159 -- guess with -0.5 on the
+  diagonal (:68-75), DIIS stored every iteration and used from iteration 3,
+  damping from iteration 2, stop on |dE| < thr.
+
+Parity: pinned by the textbook RHF energies of H2/STO-3G and the H-atom
+cc-pVDZ energy (tests/test_integrals.py) and by CD -> conventional RHF as
+the threshold goes to 0.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import jk as _jk
+
+
+def _inv_sqrt(S):
+    w, U = np.linalg.eigh(S)
+    return (U / np.sqrt(w)) @ U.T
+
+
+def _solve(F, X):
+    e, U = np.linalg.eigh(X.T @ F @ X)
+    return e, X @ U
+
+
+class _DIIS:
+    """Pulay DIIS with the bookkeeping of src/scf/diis.cc:36-108."""
+
+    def __init__(self, nmax):
+        self.nmax, self.F, self.E = nmax, [], []
+
+    def add(self, F, err):
+        self.F.append(F)
+        self.E.append(err)
+        if len(self.F) > self.nmax:
+            self.F.pop(0)
+            self.E.pop(0)
+
+    def extrapolate(self):
+        n = len(self.F)
+        A = np.zeros((n + 1, n + 1))
+        for i in range(n):
+            for j in range(n):
+                A[i, j] = np.sum(self.E[i] * self.E[j])
+        A[:n, n] = A[n, :n] = -1.0
+        b = np.zeros(n + 1)
+        b[n] = -1.0
+        c = np.linalg.lstsq(A, b, rcond=None)[0][:n]
+        return sum(ci * Fi for ci, Fi in zip(c, self.F))
+
+
+def cd_rhf(S, H, L, n_occ, e_nuc, max_iterations=100, energy_threshold=1e-8, screen=True):
+    """Returns dict(energy, C, eps, P, F, iterations, converged)."""
+    X = _inv_sqrt(S)
+    eps, C = _solve(H, X)
+    P = 2.0 * C[:, :n_occ] @ C[:, :n_occ].T
+    energy = 0.0
+    converged = False
+    F = H
+    it = 0
+    for it in range(1, max_iterations + 1):
+        e_old = energy
+        P_prev = P
+        J, K = _jk(L, P, screen=screen)
+        F = H + J - 0.5 * K
+        eps, C = _solve(F, X)
+        P_new = 2.0 * C[:, :n_occ] @ C[:, :n_occ].T
+        P = 0.7 * P_new + 0.3 * P_prev
+        energy = 0.5 * np.sum(P * (H + F).T) + e_nuc
+        de = energy - e_old if it > 1 else energy
+        if abs(de) < energy_threshold and it > 1:
+            converged = True
+            break
+    return dict(energy=energy, C=C, eps=eps, P=P, F=F, iterations=it, converged=converged)
+
+
+def cd_uhf(S, H, L, na, nb, e_nuc, max_iterations=100, energy_threshold=1e-8, diis_max=6):
+    X = _inv_sqrt(S)
+    Hb = H - 0.5 * np.eye(H.shape[0])
+    ea, Ca = _solve(Hb, X)
+    eb, Cb = _solve(Hb, X)
+    Pa = Ca[:, :na] @ Ca[:, :na].T
+    Pb = Cb[:, :nb] @ Cb[:, :nb].T
+    da, db = _DIIS(diis_max), _DIIS(diis_max)
+    energy = 0.0
+    it = 0
+    for it in range(1, max_iterations + 1):
+        e_old = energy
+        J, K = _jk(L, np.stack([Pa, Pb]))
+        Fa = H + J - K[0]
+        Fb = H + J - K[1]
+        da.add(Fa, Fa @ Pa @ S - S @ Pa @ Fa)
+        db.add(Fb, Fb @ Pb @ S - S @ Pb @ Fb)
+        if it > 2:
+            Fa, Fb = da.extrapolate(), db.extrapolate()
+        ea, Ca = _solve(Fa, X)
+        eb, Cb = _solve(Fb, X)
+        Pa_new = Ca[:, :na] @ Ca[:, :na].T
+        Pb_new = Cb[:, :nb] @ Cb[:, :nb].T
+        if it > 1:
+            Pa, Pb = 0.7 * Pa_new + 0.3 * Pa, 0.7 * Pb_new + 0.3 * Pb
+        else:
+            Pa, Pb = Pa_new, Pb_new
+        energy = 0.5 * (np.sum(Pa * (H + Fa).T) + np.sum(Pb * (H + Fb).T))
+        if abs(energy - e_old) < energy_threshold:
+            break
+    return dict(energy=energy + e_nuc, Ca=Ca, Cb=Cb, ea=ea, eb=eb, Pa=Pa, Pb=Pb, iterations=it)

@@ -1,0 +1,84 @@
+"""Host-side drop-in pieces that need no GPU: the pivoted Cholesky of
+CholeskyERI against the oracle restatement, API conventions and errors."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT
+
+BD = os.path.join(ROOT, "mshqc_amd", "data", "basis")
+
+
+@pytest.fixture(scope="module")
+def m():
+    from mshqc_amd import native_module
+
+    return native_module()
+
+
+def h_chain(m, n, spacing=1.4):
+    mol = m.Molecule()
+    for k in range(n):
+        mol.add_atom(1, 0.0, 0.0, spacing * k)
+    return mol
+
+
+def test_cholesky_eri_matches_oracle(m):
+    mol = h_chain(m, 4)
+    b = m.BasisSet("cc-pvdz", mol, BD)
+    eri = m.IntegralEngine(mol, b).compute_eri()
+    for thr in (1e-4, 1e-6):
+        ce = m.CholeskyERI(thr)
+        res = ce.decompose(eri)
+        L = np.array(ce.get_L_vectors())
+        L_ref = oracle.cholesky(eri.numpy(), thr)
+        assert res.n_vectors == ce.n_vectors() == L_ref.shape[0]
+        assert np.abs(L - L_ref).max() < 1e-12
+        n = b.n_basis_functions()
+        full = eri.numpy().reshape(n * n, n * n)
+        d = np.diag(full - L.T @ L)
+        assert d.max() < thr  # reconstruction on the diagonal
+        assert abs(ce.reconstruct(0, 1, 2, 3) - L[:, 0 * n + 1] @ L[:, 2 * n + 3]) < 1e-14
+
+
+def test_cholesky_eri_edge_cases(m):
+    with pytest.raises(ValueError):
+        m.CholeskyERI(0.0)
+    mol = h_chain(m, 2)
+    eri = m.IntegralEngine(mol, m.BasisSet("sto-3g", mol, BD)).compute_eri()
+    ce = m.CholeskyERI(1e9)  # above every diagonal: no vectors
+    assert ce.decompose(eri).n_vectors == 0
+    ce2 = m.CholeskyERI()
+    L = np.random.default_rng(0).standard_normal((3, 2, 2))
+    ce2.set_vectors(L)
+    assert ce2.n_vectors() == 3 and ce2.n_basis() == 2
+    ce2.add_vector(np.ones(4))
+    assert ce2.n_vectors() == 4
+
+
+def test_rhf_rejects_odd_electrons(m):
+    mol = h_chain(m, 3)
+    b = m.BasisSet("sto-3g", mol, BD)
+    eng = m.IntegralEngine(mol, b)
+    with pytest.raises(RuntimeError, match="closed-shell"):
+        m.CholeskyRHF(mol, b, eng, m.CholeskyRHFConfig())
+
+
+def test_eigh_matches_numpy(m):
+    A = np.random.default_rng(1).standard_normal((30, 30))
+    A = A + A.T
+    w, V = m.linalg_eigh(A)
+    assert np.allclose(w, np.linalg.eigvalsh(A), atol=1e-12)
+    assert np.allclose(V @ np.diag(w) @ V.T, A, atol=1e-11)
+
+
+def test_config_defaults_follow_reference(m):
+    c = m.CholeskyRHFConfig()
+    assert c.cholesky_threshold == 1e-6 and c.screen_exchange and c.max_iterations == 100
+    assert c.energy_threshold == 1e-8 and c.diis_max_vectors == 8
+    u = m.CholeskyUHFConfig()
+    assert u.diis_max_vectors == 6 and u.max_iterations == 100
+    r = m.CholeskyRMP2Config()
+    assert r.precision == 0 and r.cholesky_threshold == 1e-6

@@ -54,7 +54,7 @@ class _DIIS:
         A[:n, n] = A[n, :n] = -1.0
         b = np.zeros(n + 1)
         b[n] = -1.0
-        c = np.linalg.lstsq(A, b, rcond=None)[0][:n]
+        c = np.linalg.solve(A, b)[:n]  # LU, like the reference's fullPivLu
         return sum(ci * Fi for ci, Fi in zip(c, self.F))
 
 

@@ -75,17 +75,26 @@ def test_h10_cd_rhf_then_cd_rmp2(m):
 
 
 def test_cd_uhf_then_cd_ump2(m):
-    mol = chain(m, 3, mult=2)
-    b, eng, chol, L, S, H = setup(m, mol)
+    """Li atom (2 alpha, 1 beta), 6-31G: CD-UHF then CD-UMP2."""
+    mol = m.Molecule(0, 2)
+    mol.add_atom(3, 0.0, 0.0, 0.0)
+    b, eng, chol, L, S, H = setup(m, mol, basis="6-31g")
+    nb = b.n_basis_functions()
     cfg = m.CholeskyUHFConfig()
     cfg.print_level = 0
+    # a tight stop so both iterations reach the same fixed point (with the
+    # default 1e-8 they stop anywhere within that band)
+    cfg.energy_threshold = 1e-12
+    cfg.max_iterations = 300
     uhf = m.CholeskyUHF(mol, b, eng, 2, 1, cfg)
     uhf.set_cholesky_vectors(chol.get_L_vectors())
     res = uhf.compute()
-    ref = scf_ref.cd_uhf(S, H, L, 2, 1, mol.nuclear_repulsion_energy())
-    print(f"H3 CD-UHF: {res.energy_total!r} oracle {ref['energy']!r}", flush=True)
-    assert res.iterations == ref["iterations"]
+    ref = scf_ref.cd_uhf(S, H, L, 2, 1, mol.nuclear_repulsion_energy(), max_iterations=300,
+                         energy_threshold=1e-12)
+    print(f"Li CD-UHF: {res.energy_total!r} oracle {ref['energy']!r} iters {res.iterations}/{ref['iterations']}",
+          flush=True)
     assert abs(res.energy_total - ref["energy"]) < 1e-9
+    assert abs(res.energy_total - (-7.4315)) < 2e-3  # Li 6-31G UHF
     ucfg = m.CholeskyUMP2Config()
     ucfg.print_level = 0
     ump2 = m.CholeskyUMP2(res, b, eng, ucfg, chol).compute()
@@ -96,8 +105,10 @@ def test_cd_uhf_then_cd_ump2(m):
     E_ref = oracle.ump2(Ba, Bb, ea[:2], ea[2:], eb[:1], eb[1:])
     for got, want in zip((ump2.e_corr_ss_aa, ump2.e_corr_ss_bb, ump2.e_corr_os), E_ref):
         assert abs(got - want) < 1e-12
+    assert ump2.e_corr_ss_bb == 0.0  # one beta electron: no same-spin beta pairs
     assert abs(ump2.e_total - (res.energy_total + sum(E_ref))) < 1e-10
-    assert 0.75 <= uhf.compute_s_squared() < 0.8
+    assert 0.75 <= uhf.compute_s_squared() < 0.76
+    assert nb == 9
 
 
 def test_ump2_of_rhf_equals_rmp2(m):

@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--workload", default="mp2_o100")
     ap.add_argument("--cpu-pairs", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="MiB moved by one plain device copy before timing (a known-size "
+                         "kernel for scaling PMC byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -116,6 +119,14 @@ def main():
     dev = torch.from_numpy(ev).cuda()
     ctx = hip.context(0)
     W = rmp2_work(Q, o, v)
+    if args.copy_calibration:
+        nel = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {nel * 8} bytes read, {nel * 8} bytes written")
+        del src, dst
 
     results = {}
     precs = [parse_precision(args.precision)]

@@ -3,6 +3,7 @@
 // gradient contractions of the MP2 backward.
 #include "common.h"
 #include "mfma_f64.h"
+#include "tiles.h"
 
 namespace {
 
@@ -53,9 +54,11 @@ template <bool TA, bool TX>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmF64 g, double* __restrict__ part, int bper) {
     __shared__ double As[KC][TM + 1];
     __shared__ double Xs[KC][TN + 1];
+    int mt, ntile;
+    if (!grouped_tile((g.M + TM - 1) / TM, (g.N + TN - 1) / TN, mt, ntile)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int m0 = mt * TM, n0 = ntile * TN;
     int b_lo, b_hi;
     if (g.reduce_batch) {
         b_lo = blockIdx.z * bper;
@@ -167,8 +170,7 @@ __global__ __launch_bounds__(1024) void reduce_sum_kernel(const double* __restri
 int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return MSHQC_HIP_OK;
     if (g.K <= 0) return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: K must be positive");
-    const unsigned gx = (g.N + TN - 1) / TN, gy = (g.M + TM - 1) / TM;
-    if (gy > 65535u) return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: M too large");
+    const unsigned gx = grouped_grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN);
     int splits = 1, bper = 1;
     double* part = nullptr;
     if (g.reduce_batch) {
@@ -181,7 +183,9 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
             if (!part) return MSHQC_HIP_ERR_ALLOC;
         }
     }
-    dim3 grid(gx, gy, g.reduce_batch ? splits : g.batch);
+    const unsigned gz = g.reduce_batch ? splits : g.batch;
+    if (gz > 65535u) return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: batch too large");
+    dim3 grid(gx, 1, gz);
     if (g.transA) {
         if (g.transX) gemm_f64_kernel<true, true><<<grid, NT, 0, ctx->stream>>>(g, part, bper);
         else gemm_f64_kernel<true, false><<<grid, NT, 0, ctx->stream>>>(g, part, bper);

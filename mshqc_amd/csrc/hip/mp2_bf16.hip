@@ -15,13 +15,19 @@
 #include "mp2_bf16.h"
 #include "mp2_internal.h"
 #include "ozaki.h"
+#include "tiles.h"
 
 namespace {
 
 constexpr int QB = 128;   // Q block: exponent block and exact-FP32 span of the pair kernel
 constexpr int KC = 32;    // contraction chunk (one MFMA K step)
 constexpr int KBG = 64;   // k block of the gradient GEMM operands
-constexpr int LROW = 80;  // LDS bytes per 32-element BF16 row (64 data + 16 pad)
+constexpr int LROW = 64;  // LDS bytes per 32-element BF16 row
+
+// LDS byte offset of 16-byte segment `sg` of panel row `row`: the segment is
+// XOR-swizzled by (row >> 1) & 3, which makes the 16-lane fragment reads and
+// the 4-lanes-per-row stores free of bank conflicts (measured on gfx950).
+__device__ __forceinline__ int lds_off(int row, int sg) { return row * LROW + ((sg ^ ((row >> 1) & 3)) << 4); }
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -162,7 +168,7 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
 // 16-byte LDS fragment read: row `row` of a [64 rows][32 k] BF16 panel, the
 // 8 elements k = 8*seg .. 8*seg + 7.
 __device__ __forceinline__ v8s lds_frag(const char* panel, int row, int seg) {
-    return *reinterpret_cast<const v8s*>(panel + row * LROW + seg * 16);
+    return *reinterpret_cast<const v8s*>(panel + lds_off(row, seg));
 }
 
 // Class sums of one KC chunk for a 32x32 wave tile: rows from panel pa,
@@ -276,8 +282,8 @@ __global__ __launch_bounds__(MODE == MODE_OS ? 256 : 512) void pair_bf16_kernel(
             const int g = tid + u * NT;
             if (g < NSEG) {
                 const int P = g / (S * 256), sl = (g / 256) % S, row = (g & 255) >> 2, sg = g & 3;
-                *reinterpret_cast<uint4*>(lds + P * PSTRIDE + (long)sl * 64 * LROW + row * LROW +
-                                          sg * 16) = reg[u];
+                *reinterpret_cast<uint4*>(lds + P * PSTRIDE + (long)sl * 64 * LROW + lds_off(row, sg)) =
+                    reg[u];
             }
         }
     };
@@ -381,9 +387,11 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
                                                            long ldc) {
     constexpr long PSTRIDE = (long)S * 64 * LROW;
     extern __shared__ __align__(16) char lds[];
+    int mt, ntile;
+    if (!grouped_tile((A.rows + 63) / 64, (X.rows + 63) / 64, mt, ntile)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int m0 = mt * 64, n0 = ntile * 64;
     constexpr int NSEG = 2 * S * 256;
     constexpr int PER = (NSEG + 255) / 256;
     uint4 reg[PER];
@@ -408,8 +416,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
             const int g = tid + u * 256;
             if (g < NSEG) {
                 const int P = g / (S * 256), sl = (g / 256) % S, row = (g & 255) >> 2, sg = g & 3;
-                *reinterpret_cast<uint4*>(lds + P * PSTRIDE + (long)sl * 64 * LROW + row * LROW +
-                                          sg * 16) = reg[u];
+                *reinterpret_cast<uint4*>(lds + P * PSTRIDE + (long)sl * 64 * LROW + lds_off(row, sg)) =
+                    reg[u];
             }
         }
     };
@@ -517,7 +525,9 @@ struct PairLaunch {
     static int run(mshqc_hip_ctx* ctx, int mode, const PairArgs& a, const Bf16Operands& o) {
         const long nwg = ((a.nwork + 7) / 8) * 8;
         dim3 grid((unsigned)nwg);
-        const size_t lds_sym = (size_t)4 * S * 64 * LROW, lds_os = (size_t)2 * S * 64 * LROW;
+        // The symmetric kernel reuses its panels for the 64 x 65 FP64 H tile.
+        const size_t lds_sym = std::max((size_t)4 * S * 64 * LROW, (size_t)64 * 65 * 8);
+        const size_t lds_os = (size_t)2 * S * 64 * LROW;
         const uint16_t* sr = o.sr ? o.sr : o.sl;
         const int8_t* xr = o.sr ? o.xr : o.xl;
         switch (mode) {
@@ -541,7 +551,7 @@ struct PairLaunch {
 template <int S>
 struct GemmLaunch {
     static int run(mshqc_hip_ctx* ctx, const SlicedMat& A, const SlicedMat& X, int K, double* C, long ldc) {
-        dim3 grid(ceil_div(X.rows, 64), ceil_div(A.rows, 64));
+        dim3 grid(grouped_grid(ceil_div(A.rows, 64), ceil_div(X.rows, 64)));
         const size_t lds = (size_t)2 * S * 64 * LROW;
         gemm_bf16_nt_kernel<S><<<grid, 256, lds, ctx->stream>>>(A, X, K, C, ldc);
         LAUNCH_CHECK(ctx);
@@ -553,7 +563,7 @@ template <int S>
 struct AllowLds {
     static int run(mshqc_hip_ctx* ctx) {
         // Dynamic LDS above 64 KiB must be opted into per kernel.
-        const int big = 4 * S * 64 * LROW;
+        const int big = std::max(4 * S * 64 * LROW, 64 * 65 * 8);
         hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_RMP2, S>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, big);
         hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_SS, S>,

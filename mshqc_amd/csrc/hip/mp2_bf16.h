@@ -22,7 +22,7 @@ struct Bf16Operands {
     const uint16_t* sr = nullptr;
     const int8_t* xr = nullptr;
     // Gradient-GEMM operands: slices [S][Q][o*vp] with exponents per
-    // (q, i, 64-wide a block), left and right.
+    // (q, i, 128-wide a block), left and right.
     const uint16_t* gl = nullptr;
     const int8_t* gel = nullptr;
     const uint16_t* gr = nullptr;

@@ -7,7 +7,8 @@
 //                       denominator epilogue (same one as the FP64 kernel)
 //   slice_rows_kernel / slice_cols_kernel
 //                       row- and column-blocked slices of the gradient GEMM
-//                       operands (B and the amplitude block Z, Z^T)
+//                       operands (B and the amplitude block Z, Z^T), one
+//                       exponent per row and 128-wide k block
 //   gemm_bf16_nt_kernel C[m][n] += sum_k A[m][k] X[n][k] from slices
 // The arithmetic contract is described in ozaki.h.
 #include <algorithm>
@@ -21,8 +22,9 @@ namespace {
 
 constexpr int QB = 128;   // Q block: exponent block and exact-FP32 span of the pair kernel
 constexpr int KC = 32;    // contraction chunk (one MFMA K step)
-constexpr int KBG = 64;   // k block of the gradient GEMM operands
 constexpr int LROW = 64;  // LDS bytes per 32-element BF16 row
+constexpr int GB = 128;   // exponent block of the gradient-GEMM operands along k
+constexpr int GT = 128;   // gradient-GEMM workgroup tile (rows and columns)
 
 // LDS byte offset of 16-byte segment `sg` of panel row `row`: the segment is
 // XOR-swizzled by (row >> 1) & 3, which makes the 16-lane fragment reads and
@@ -81,14 +83,14 @@ __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restric
 }
 
 // src[R][ld] with column index seg*v + b (seg < nseg, b < v)  ->
-// dst[S][R][nseg*vp] (column seg*vp + b, zero padded), ex[R][nseg*(vp/64)].
-// One wave per (row, seg, 64-wide b tile).
+// dst[S][R][nseg*vp] (column seg*vp + b, zero padded), ex[R][nseg*(vp/GB)].
+// One wave per (row, seg, GB-wide b tile); each lane slices two elements.
 template <int S>
 __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restrict__ src, long ld, int R,
                                                          int nseg, int v, int vp,
                                                          uint16_t* __restrict__ dst,
                                                          int8_t* __restrict__ ex) {
-    const int ntv = vp / 64;
+    const int ntv = vp / GB;
     const long job = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const long njobs = (long)R * nseg * ntv;
     if (job >= njobs) return;
@@ -96,54 +98,64 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
     const int row = (int)(job / (nseg * ntv));
     const int rem = (int)(job % (nseg * ntv));
     const int seg = rem / ntv, tile = rem % ntv;
-    const int b = tile * 64 + lane;
-    const double x = b < v ? src[(long)row * ld + (long)seg * v + b] : 0.0;
-    double mx = fabs(x);
+    const int b0 = tile * GB + lane, b1 = b0 + 64;
+    const double* s = src + (long)row * ld + (long)seg * v;
+    const double x0 = b0 < v ? s[b0] : 0.0;
+    const double x1 = b1 < v ? s[b1] : 0.0;
+    double mx = fmax(fabs(x0), fabs(x1));
     for (int m = 1; m < 64; m <<= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
     const int e = block_exponent(mx);
-    uint16_t sl[S];
-    slice_value<S>(x, e, sl);
+    uint16_t s0[S], s1[S];
+    slice_value<S>(x0, e, s0);
+    slice_value<S>(x1, e, s1);
     const long kw = (long)nseg * vp;
     const long slice = (long)R * kw;
-    uint16_t* d = dst + (long)row * kw + (long)seg * vp + b;
+    uint16_t* d = dst + (long)row * kw + (long)seg * vp + tile * GB + lane;
 #pragma unroll
-    for (int p = 0; p < S; ++p) d[p * slice] = sl[p];
+    for (int p = 0; p < S; ++p) {
+        d[p * slice] = s0[p];
+        d[p * slice + 64] = s1[p];
+    }
     if (lane == 0) ex[(long)row * nseg * ntv + rem] = (int8_t)e;
 }
 
 // Transposed slicing of Z[(i',a)][(j',b)] (rows i'*v + a, i' < nI; ld = ldz;
 // nc = nJ*vr columns):  dst[S][nc][nI*vp] with row n = column of Z and
-// column i'*vp + a;  ex[nc][nI*(vp/64)].  Tiles go through LDS.
+// column i'*vp + a;  ex[nc][nI*(vp/GB)].  A workgroup transposes a tile of
+// 32 columns x GB values of a through LDS.
 template <int S>
 __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restrict__ Z, long ldz, int nI,
                                                          int v, int vp, int nc,
                                                          uint16_t* __restrict__ dst,
                                                          int8_t* __restrict__ ex) {
-    __shared__ double T[64][65];
-    __shared__ int exs[64];
-    const int ntv = vp / 64;
-    const int c0 = blockIdx.x * 64;
-    const int ip = blockIdx.y / ntv, at = blockIdx.y % ntv, a0 = at * 64;
+    __shared__ double T[GB][33];
+    __shared__ int exs[32];
+    const int ntv = vp / GB;
+    const int c0 = blockIdx.x * 32;
+    const int ip = blockIdx.y / ntv, at = blockIdx.y % ntv, a0 = at * GB;
     const int tid = threadIdx.x;
     {
-        const int cc = tid & 63;
-        for (int k = 0; k < 16; ++k) {
-            const int aa = (tid >> 6) + 4 * k;
+        const int cc = tid & 31;
+        for (int k = 0; k < GB / 8; ++k) {
+            const int aa = (tid >> 5) + 8 * k;
             double x = 0.0;
             if (a0 + aa < v && c0 + cc < nc) x = Z[((long)ip * v + a0 + aa) * ldz + c0 + cc];
             T[aa][cc] = x;
         }
     }
     __syncthreads();
-    if (tid < 64) {
+    {
+        // 8 lanes per column, 16 values each, then a shuffle maximum.
+        const int cc = tid >> 3, part = tid & 7;
         double mx = 0.0;
-        for (int aa = 0; aa < 64; ++aa) mx = fmax(mx, fabs(T[aa][tid]));
-        exs[tid] = block_exponent(mx);
+        for (int k = 0; k < GB / 8; ++k) mx = fmax(mx, fabs(T[part * (GB / 8) + k][cc]));
+        for (int m = 1; m < 8; m <<= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
+        if (part == 0) exs[cc] = block_exponent(mx);
     }
     __syncthreads();
     const long kw = (long)nI * vp, slice = (long)nc * kw;
     // thread -> (column cc, 16 consecutive a)
-    const int cc = tid >> 2, part = tid & 3;
+    const int cc = tid >> 3, part = tid & 7;
     if (c0 + cc >= nc) return;
     const int e = exs[cc];
     uint16_t* d = dst + (long)(c0 + cc) * kw + (long)ip * vp + a0 + 16 * part;
@@ -381,94 +393,142 @@ struct SlicedMat {
     int rows;
 };
 
-// C[m][n] += sum_k A[m][k] X[n][k]; K is a multiple of KBG; 64x64 tiles.
+// Wait for this wave's outstanding vector-memory operations (the LDS DMA
+// loads), then a workgroup barrier.
+__device__ __forceinline__ void dma_barrier() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __syncthreads();
+}
+
+// Direct global -> LDS load of 16 BF16 panel rows x 64 bytes by one wave.
+// The instruction writes lane l's 16 bytes at dst + 16*l, i.e. panel row
+// l/4 of the 16, physical segment l&3; lane_seg() is the logical segment
+// that lds_off() places there.  `dst` must be wave-uniform.
+__device__ __forceinline__ int lane_seg(int lane) { return (lane & 3) ^ ((lane >> 3) & 3); }
+
+__device__ __forceinline__ void dma16(const uint16_t* src, char* dst) {
+    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
+// C[m][n] += sum_k A[m][k] X[n][k] for the gradient contractions; K is a
+// multiple of GB.  128 x 128 workgroup tile, four waves of 64 x 64, one
+// 32-wide k step per stage; stages are loaded straight into a double-buffered
+// LDS by the DMA path (one barrier per stage, no staging registers).  The
+// gradient needs ~1e-7 relative accuracy, so products go to two FP32 tiers:
+// c = 0 (exact integers) and c >= 1 (rounding near 2^-31 of the block
+// scale); both are flushed to FP64 with the block exponents once per GB-wide
+// k block.
 template <int S>
 __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMat X, int K, double* __restrict__ C,
                                                            long ldc) {
-    constexpr long PSTRIDE = (long)S * 64 * LROW;
+    constexpr long PANEL = (long)GT * LROW;   // bytes of one slice panel (128 rows x 32 k)
+    constexpr long STAGE = 2L * S * PANEL;    // both operands, all slices
+    constexpr int PER = 2 * S * (GT / 16) / 4; // 16-row DMA loads per wave and stage
     extern __shared__ __align__(16) char lds[];
     int mt, ntile;
-    if (!grouped_tile((A.rows + 63) / 64, (X.rows + 63) / 64, mt, ntile)) return;
+    if (!grouped_tile((A.rows + GT - 1) / GT, (X.rows + GT - 1) / GT, mt, ntile)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    const int m0 = mt * 64, n0 = ntile * 64;
-    constexpr int NSEG = 2 * S * 256;
-    constexpr int PER = (NSEG + 255) / 256;
-    uint4 reg[PER];
-    auto fetch = [&](int k0) {
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int m0 = mt * GT, n0 = ntile * GT;
+    // DMA load u of this wave covers operand P, slice sl and panel rows
+    // 16*rb .. 16*rb+15 (id = wave + 4u enumerates (P, sl, rb)); lanes of rows
+    // past the matrix load nothing, and those LDS rows only feed discarded
+    // outputs.
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int lrow = lane >> 2;
+    const long laneA = (long)lrow * A.ld + 8 * lane_seg(lane);
+    const long laneX = (long)lrow * X.ld + 8 * lane_seg(lane);
+    auto issue = [&](int k0, int buf) {
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
-            const int g = tid + u * 256;
-            uint4 val = make_uint4(0, 0, 0, 0);
-            if (g < NSEG) {
-                const int P = g / (S * 256), sl = (g / 256) % S, row = (g & 255) >> 2, sg = g & 3;
-                const SlicedMat& M = P ? X : A;
-                const int r = (P ? n0 : m0) + row;
-                if (r < M.rows)
-                    val = *reinterpret_cast<const uint4*>(M.s + sl * M.plane + (long)r * M.ld + k0 + 8 * sg);
-            }
-            reg[u] = val;
+            const int id = wv + 4 * u;
+            const int P = id / (8 * S), sl = (id / 8) % S, rb = id % 8;
+            const SlicedMat& M = P ? X : A;
+            const int row = (P ? n0 : m0) + 16 * rb;
+            if (row + lrow < M.rows)
+                dma16(M.s + sl * M.plane + (long)row * M.ld + k0 + (P ? laneX : laneA),
+                      lds + buf * STAGE + (P * S + sl) * PANEL + 1024 * rb);
         }
     };
-    auto stash = [&]() {
+    v4f acc[2][4][4];
+    v4d acc64[4][4];
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int g = tid + u * 256;
-            if (g < NSEG) {
-                const int P = g / (S * 256), sl = (g / 256) % S, row = (g & 255) >> 2, sg = g & 3;
-                *reinterpret_cast<uint4*>(lds + P * PSTRIDE + (long)sl * 64 * LROW + lds_off(row, sg)) =
-                    reg[u];
-            }
-        }
-    };
-    v4f acc[3][2][2];
-    v4d acc64[2][2];
+    for (int s = 0; s < 4; ++s)
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < 4; ++t) {
             acc64[s][t] = v4d{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[c][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
+            acc[0][s][t] = acc[1][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
         }
-    constexpr bool PREFETCH = true;
-    const int nchunk = K / KC;
-    if (PREFETCH) fetch(0);
-    for (int ch = 0; ch < nchunk; ++ch) {
-        __syncthreads();
-        if (!PREFETCH) fetch(ch * KC);
-        stash();
-        __syncthreads();
-        if (PREFETCH && ch + 1 < nchunk) fetch((ch + 1) * KC);
-        chunk_mfma<S>(lds, lds + PSTRIDE, (long)64 * LROW, wm, wn, lane, acc);
-        if ((ch + 1) % (KBG / KC) == 0) {
-            const int kb = ch / (KBG / KC);
-            int eA[2][4], eB[2];
+    const int rr = lane & 15, fs = lane >> 4;
+    const int nkb = K / GB;
+    constexpr int CPB = GB / KC;  // chunks per exponent block (even: buffer = chunk parity)
+    issue(0, 0);
+    dma_barrier();
+    for (int kb = 0; kb < nkb; ++kb) {
+        // Exponents of this block, fetched early; used by the flush.
+        int eA[4][4], eB[4];
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = m0 + wm + 16 * s + bf16_row(lane, r);
-                    eA[s][r] = m < A.rows ? A.e[(long)m * A.lde + kb] : 0;
-                }
-#pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int n = n0 + wn + 16 * t + (lane & 15);
-                eB[t] = n < X.rows ? X.e[(long)n * X.lde + kb] : 0;
-            }
-            flush(acc, acc64, eA, eB);
+        for (int t = 0; t < 4; ++t) {
+            const int n = n0 + wn + 16 * t + rr;
+            eB[t] = n < X.rows ? X.e[(long)n * X.lde + kb] : 0;
         }
-    }
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
+        for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm + 16 * s + bf16_row(lane, r);
-                const int n = n0 + wn + 16 * t + (lane & 15);
-                if (m < A.rows && n < X.rows) C[(long)m * ldc + n] += acc64[s][t][r];
+                const int m = m0 + wm + 16 * s + 4 * fs + r;
+                eA[s][r] = m < A.rows ? A.e[(long)m * A.lde + kb] : 0;
             }
+#pragma unroll
+        for (int c = 0; c < CPB; ++c) {
+            const int buf = c & 1;
+            if (kb * CPB + c + 1 < nkb * CPB) issue((kb * CPB + c + 1) * KC, buf ^ 1);
+            const char* pa = lds + buf * STAGE;
+            const char* pb = pa + S * PANEL;
+            v8s b[S][4];
+#pragma unroll
+            for (int q = 0; q < S; ++q)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) b[q][t] = lds_frag(pb + q * PANEL, wn + 16 * t + rr, fs);
+#pragma unroll
+            for (int p = 0; p < S; ++p) {
+                v8s a[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) a[s] = lds_frag(pa + p * PANEL, wm + 16 * s + rr, fs);
+#pragma unroll
+                for (int q = 0; q < S - p; ++q)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            acc[p + q ? 1 : 0][s][t] = mfma_bf16(a[s], b[q][t], acc[p + q ? 1 : 0][s][t]);
+            }
+            dma_barrier();
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double sum = (double)acc[0][s][t][r] + (double)acc[1][s][t][r];
+                    acc64[s][t][r] += sum * pow2(eA[s][r] + eB[t] - 16);
+                }
+                acc[0][s][t] = acc[1][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
+            }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = m0 + wm + 16 * s + 4 * fs + r;
+            if (m >= A.rows) continue;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int n = n0 + wn + 16 * t + rr;
+                if (n < X.rows) C[(long)m * ldc + n] += acc64[s][t][r];
+            }
+        }
 }
 
 // ------------------------------------------------------------ dispatch
@@ -485,7 +545,21 @@ int by_slices(int S, Args&&... args) {
     }
 }
 
+// Gradient-GEMM operands carry at most BF16_GRAD_SLICES slices.
+template <template <int> class F, typename... Args>
+int by_gslices(int S, Args&&... args) {
+    static_assert(BF16_GRAD_SLICES == 3, "instantiate the gradient slice counts");
+    switch (S) {
+        case 2: return F<2>::run(args...);
+        case 3: return F<3>::run(args...);
+        default: return -1;
+    }
+}
+
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Dynamic LDS of the gradient GEMM: two stages of both operands' slices.
+constexpr size_t gemm_lds(int S) { return (size_t)2 * 2 * S * GT * LROW; }
 
 template <int S>
 struct SlicePair {
@@ -502,7 +576,7 @@ template <int S>
 struct SliceRows {
     static int run(mshqc_hip_ctx* ctx, const double* src, long ld, int R, int nseg, int v, int vp,
                    uint16_t* dst, int8_t* ex) {
-        const long jobs = (long)R * nseg * (vp / 64);
+        const long jobs = (long)R * nseg * (vp / GB);
         slice_rows_kernel<S><<<ceil_div(jobs, 4), 256, 0, ctx->stream>>>(src, ld, R, nseg, v, vp, dst, ex);
         LAUNCH_CHECK(ctx);
         return 0;
@@ -513,7 +587,7 @@ template <int S>
 struct SliceCols {
     static int run(mshqc_hip_ctx* ctx, const double* Z, long ldz, int nI, int v, int vp, int nc,
                    uint16_t* dst, int8_t* ex) {
-        dim3 grid(ceil_div(nc, 64), nI * (vp / 64));
+        dim3 grid(ceil_div(nc, 32), nI * (vp / GB));
         slice_cols_kernel<S><<<grid, 256, 0, ctx->stream>>>(Z, ldz, nI, v, vp, nc, dst, ex);
         LAUNCH_CHECK(ctx);
         return 0;
@@ -551,9 +625,10 @@ struct PairLaunch {
 template <int S>
 struct GemmLaunch {
     static int run(mshqc_hip_ctx* ctx, const SlicedMat& A, const SlicedMat& X, int K, double* C, long ldc) {
-        dim3 grid(grouped_grid(ceil_div(A.rows, 64), ceil_div(X.rows, 64)));
-        const size_t lds = (size_t)2 * S * 64 * LROW;
-        gemm_bf16_nt_kernel<S><<<grid, 256, lds, ctx->stream>>>(A, X, K, C, ldc);
+        if (K % GB != 0)
+            return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 GEMM: K=%d is not a multiple of %d", K, GB);
+        dim3 grid(grouped_grid(ceil_div(A.rows, GT), ceil_div(X.rows, GT)));
+        gemm_bf16_nt_kernel<S><<<grid, 256, gemm_lds(S), ctx->stream>>>(A, X, K, C, ldc);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -570,8 +645,17 @@ struct AllowLds {
                             hipFuncAttributeMaxDynamicSharedMemorySize, big);
         hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_OS, S>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, big / 2);
+        hipGetLastError();
+        (void)ctx;
+        return 0;
+    }
+};
+
+template <int S>
+struct AllowLdsGemm {
+    static int run(mshqc_hip_ctx* ctx) {
         hipFuncSetAttribute((const void*)gemm_bf16_nt_kernel<S>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, big / 2);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)gemm_lds(S));
         hipGetLastError();
         (void)ctx;
         return 0;
@@ -587,6 +671,7 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     Bf16Operands o;
     o.slices = S;
     o.gslices = std::min(S, BF16_GRAD_SLICES);
+    by_gslices<AllowLdsGemm>(o.gslices, ctx);
     o.Q = Q;
     o.Qp = round_up(Q, QB);
     o.qblocks = o.Qp / QB;
@@ -610,22 +695,22 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     }
     if (grad) {
         // GEMM-operand slices of B: rows q, columns (i, a) blocked by 64 along a.
-        o.vlp = round_up(vl, 64);
-        o.vrp = round_up(vr, 64);
+        o.vlp = round_up(vl, GB);
+        o.vrp = round_up(vr, GB);
         const int SG = o.gslices;
         const size_t gl = (size_t)SG * Q * ol * o.vlp * 2, gr = Br ? (size_t)SG * Q * orr * o.vrp * 2 : 0;
-        const size_t gel = (size_t)Q * ol * (o.vlp / 64), ger = Br ? (size_t)Q * orr * (o.vrp / 64) : 0;
+        const size_t gel = (size_t)Q * ol * (o.vlp / GB), ger = Br ? (size_t)Q * orr * (o.vrp / GB) : 0;
         char* g = static_cast<char*>(workspace(ctx, WS_SLICE2, gl + gr + gel + ger + 1024));
         if (!g) return MSHQC_HIP_ERR_ALLOC;
         o.gl = reinterpret_cast<uint16_t*>(g);
         o.gr = Br ? reinterpret_cast<uint16_t*>(g + gl) : o.gl;
         o.gel = reinterpret_cast<int8_t*>(g + gl + gr);
         o.ger = Br ? reinterpret_cast<int8_t*>(g + gl + gr + gel) : o.gel;
-        rc = by_slices<SliceRows>(SG, ctx, Bl, (long)ovl, Q, ol, vl, o.vlp, const_cast<uint16_t*>(o.gl),
+        rc = by_gslices<SliceRows>(SG, ctx, Bl, (long)ovl, Q, ol, vl, o.vlp, const_cast<uint16_t*>(o.gl),
                                   const_cast<int8_t*>(o.gel));
         if (rc) return rc;
         if (Br) {
-            rc = by_slices<SliceRows>(SG, ctx, Br, (long)ovr, Q, orr, vr, o.vrp, const_cast<uint16_t*>(o.gr),
+            rc = by_gslices<SliceRows>(SG, ctx, Br, (long)ovr, Q, orr, vr, o.vrp, const_cast<uint16_t*>(o.gr),
                                       const_cast<int8_t*>(o.ger));
             if (rc) return rc;
         }
@@ -651,33 +736,33 @@ int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& o, const double* 
     if (dBl) {
         // dBl[q][(i,a)] += sum_{(j,b)} Br[q][(j,b)] Z[(i,a)][(j,b)]
         const long R = (long)nI * vl, kw = (long)nJ * vrp;
-        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * nJ * (vrp / 64);
+        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * nJ * (vrp / GB);
         char* w = static_cast<char*>(workspace(ctx, WS_ZS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;
         uint16_t* zs = reinterpret_cast<uint16_t*>(w);
         int8_t* zx = reinterpret_cast<int8_t*>(w + zb);
-        rc = by_slices<SliceRows>(S, ctx, Z, ldz, (int)R, nJ, vr, vrp, zs, zx);
+        rc = by_gslices<SliceRows>(S, ctx, Z, ldz, (int)R, nJ, vr, vrp, zs, zx);
         if (rc) return rc;
         SlicedMat A{o.gr + (long)J0 * vrp, (long)Q * o.orr * vrp, (long)o.orr * vrp,
-                    o.ger + (long)J0 * (vrp / 64), (long)o.orr * (vrp / 64), Q};
-        SlicedMat X{zs, (long)R * kw, kw, zx, (long)nJ * (vrp / 64), (int)R};
-        rc = by_slices<GemmLaunch>(S, ctx, A, X, (int)kw, dBl + (long)I0 * vl, (long)o.ol * vl);
+                    o.ger + (long)J0 * (vrp / GB), (long)o.orr * (vrp / GB), Q};
+        SlicedMat X{zs, (long)R * kw, kw, zx, (long)nJ * (vrp / GB), (int)R};
+        rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBl + (long)I0 * vl, (long)o.ol * vl);
         if (rc) return rc;
     }
     if (second && dBr) {
         // dBr[q][(j,b)] += sum_{(i,a)} Bl[q][(i,a)] Z[(i,a)][(j,b)]
         const long R = (long)nJ * vr, kw = (long)nI * vlp;
-        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * nI * (vlp / 64);
+        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * nI * (vlp / GB);
         char* w = static_cast<char*>(workspace(ctx, WS_ZTS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;
         uint16_t* zs = reinterpret_cast<uint16_t*>(w);
         int8_t* zx = reinterpret_cast<int8_t*>(w + zb);
-        rc = by_slices<SliceCols>(S, ctx, Z, ldz, nI, vl, vlp, (int)R, zs, zx);
+        rc = by_gslices<SliceCols>(S, ctx, Z, ldz, nI, vl, vlp, (int)R, zs, zx);
         if (rc) return rc;
         SlicedMat A{o.gl + (long)I0 * vlp, (long)Q * o.ol * vlp, (long)o.ol * vlp,
-                    o.gel + (long)I0 * (vlp / 64), (long)o.ol * (vlp / 64), Q};
-        SlicedMat X{zs, (long)R * kw, kw, zx, (long)nI * (vlp / 64), (int)R};
-        rc = by_slices<GemmLaunch>(S, ctx, A, X, (int)kw, dBr + (long)J0 * vr, (long)o.orr * vr);
+                    o.gel + (long)I0 * (vlp / GB), (long)o.ol * (vlp / GB), Q};
+        SlicedMat X{zs, (long)R * kw, kw, zx, (long)nI * (vlp / GB), (int)R};
+        rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBr + (long)J0 * vr, (long)o.orr * vr);
         if (rc) return rc;
     }
     return MSHQC_HIP_OK;

@@ -89,8 +89,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", help="fp64 | bf16xK (K slices)")
-    ap.add_argument("--compare", default="", help="second precision to time beside it")
+    ap.add_argument("--precision", default="bf16x5", help="fp64 | bf16xK (K slices)")
+    ap.add_argument("--compare", default="fp64", help="second precision timed beside it ('' for none)")
     ap.add_argument("--workload", default="mp2_o100")
     ap.add_argument("--cpu-pairs", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")

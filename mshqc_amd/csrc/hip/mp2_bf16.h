@@ -15,8 +15,10 @@ struct Bf16Operands {
     int gslices = 0;  // slices of the gradient-GEMM operands
     int Q = 0, Qp = 0, qblocks = 0;
     int ol = 0, vl = 0, orr = 0, vr = 0, vlp = 0, vrp = 0;
-    // Pair-kernel operands: slices [S][o*v][Qp] (bf16 bits) and exponents
-    // [o*v][Qp/128], left and right (right = left in the symmetric modes).
+    int vEl = 0, vEr = 0;  // row stride of the pair-kernel exponent tables
+    // Pair-kernel operands: slices [S][Qp/16][o*v][16] (bf16 bits) and
+    // exponents [Qp/128][o][vE], left and right (right = left in the
+    // symmetric modes).
     const uint16_t* sl = nullptr;
     const int8_t* xl = nullptr;
     const uint16_t* sr = nullptr;

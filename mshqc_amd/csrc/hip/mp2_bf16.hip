@@ -1,10 +1,11 @@
 // Split-BF16 form of the MP2 kernels on gfx950 BF16 matrix cores.
 //
-//   slice_pair_kernel   B[Q][o][v] -> BF16 slices [S][o*v][Qp] with one exponent
-//                       per (i,a) and 128-wide Q block (per-Q-block scaling)
-//   pair_bf16_kernel    (ia|jb) tiles from the slices, exact FP32 class sums per
-//                       Q block, FP64 across blocks, then the shared FP64
-//                       denominator epilogue (same one as the FP64 kernel)
+//   slice_pair_kernel   B[Q][o*v] -> BF16 slices [S][Qp/16][o*v][16] (each
+//                       16-wide Q step of 32 rows is one contiguous KiB) and
+//                       one exponent per (i, a) and 128-wide Q block
+//   pair_bf16_kernel    (ia|jb) tiles from the slices on 32x32x16 MFMAs, exact
+//                       FP32 class sums per Q block, FP64 across blocks, then
+//                       the FP64 denominator epilogue
 //   slice_rows_kernel / slice_cols_kernel
 //                       row- and column-blocked slices of the gradient GEMM
 //                       operands (B and the amplitude block Z, Z^T), one
@@ -21,7 +22,9 @@
 namespace {
 
 constexpr int QB = 128;   // Q block: exponent block and exact-FP32 span of the pair kernel
-constexpr int KC = 32;    // contraction chunk (one MFMA K step)
+constexpr int KS = 16;    // Q per pair-kernel stage (one 32x32x16 MFMA K step)
+constexpr int RB = 32;    // LDS bytes per panel row, slice and pair-kernel stage
+constexpr int KC = 32;    // contraction chunk of the gradient GEMM (one 16x16x32 MFMA K step)
 constexpr int LROW = 64;  // LDS bytes per 32-element BF16 row
 constexpr int GB = 128;   // exponent block of the gradient-GEMM operands along k
 constexpr int GT = 128;   // gradient-GEMM workgroup tile (rows and columns)
@@ -31,16 +34,25 @@ constexpr int GT = 128;   // gradient-GEMM workgroup tile (rows and columns)
 // the 4-lanes-per-row stores free of bank conflicts (measured on gfx950).
 __device__ __forceinline__ int lds_off(int row, int sg) { return row * LROW + ((sg ^ ((row >> 1) & 3)) << 4); }
 
+// LDS byte offset of 16-byte segment `seg` (0, 1) of a 32-byte pair-kernel
+// panel row: the segment is swapped on bit 2 xor bit 3 of the row, which
+// keeps the 32-lane fragment reads free of bank conflicts for 32 or 64 banks.
+__device__ __forceinline__ int lds_off32(int row, int seg) {
+    return row * RB + ((seg ^ (((row >> 2) ^ (row >> 3)) & 1)) << 4);
+}
+
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
 // ------------------------------------------------------------- slicing
 
-// B[Q][ov] (row-major, rows = q) -> Ps[S][ov][Qp], Pe[ov][Qb].  One workgroup
-// per 64 columns x 128 q; the tile is transposed through LDS so both the
-// FP64 reads and the BF16 writes are contiguous.
+// B[Q][o*v] (row-major, rows = q) -> Ps[S][Qp/16][o*v][16] and
+// Pe[Qp/128][o][vE] (exponent of row (i, a) over one 128-wide Q block; the
+// caller zeroes Pe, so the padding a >= v reads 0).  One workgroup per 64
+// rows x 128 q; the tile is transposed through LDS so both the FP64 reads and
+// the BF16 writes are contiguous.
 template <int S>
-__global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restrict__ B, int Q, int ov,
-                                                         int Qp, int Qb, uint16_t* __restrict__ Ps,
+__global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restrict__ B, int Q, int ov, int v,
+                                                         int vE, int o, int Qp, uint16_t* __restrict__ Ps,
                                                          int8_t* __restrict__ Pe) {
     __shared__ double T[QB][65];
     __shared__ int ex[64];
@@ -59,27 +71,30 @@ __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restric
         double mx = 0.0;
         for (int qq = 0; qq < QB; ++qq) mx = fmax(mx, fabs(T[qq][tid]));
         ex[tid] = block_exponent(mx);
+        const int r = r0 + tid;
+        if (r < ov) Pe[((long)blockIdx.y * o + r / v) * vE + r % v] = (int8_t)ex[tid];
     }
     __syncthreads();
-    const int r = tid >> 2, seg = tid & 3;  // row r, q in [32 seg, 32 seg + 32)
-    if (r0 + r >= ov) return;
-    const int e = ex[r];
-    const long slice = (long)ov * Qp;
-    uint16_t* dst = Ps + (long)(r0 + r) * Qp + q0 + 32 * seg;
-    for (int q8 = 0; q8 < 32; q8 += 8) {
+    const long plane = (long)Qp * ov;
+    // Item (q16 block kq, row, half): 8 consecutive q of one row; consecutive
+    // threads write consecutive 16-byte pieces of the 2 KiB (64 rows x 32 B)
+    // block of one 16-wide Q step.
+    for (int idx = tid; idx < (QB / KS) * 64 * 2; idx += 256) {
+        const int kq = idx >> 7, row = (idx & 127) >> 1, half = idx & 1;
+        if (r0 + row >= ov) continue;
+        const int e = ex[row];
         uint16_t out[S][8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             uint16_t m[S];
-            slice_value<S>(T[32 * seg + q8 + u][r], e, m);
+            slice_value<S>(T[kq * KS + half * 8 + u][row], e, m);
 #pragma unroll
             for (int p = 0; p < S; ++p) out[p][u] = m[p];
         }
+        uint16_t* dst = Ps + ((long)(q0 / KS + kq) * ov + r0 + row) * KS + half * 8;
 #pragma unroll
-        for (int p = 0; p < S; ++p)
-            *reinterpret_cast<uint4*>(dst + p * slice + q8) = *reinterpret_cast<uint4*>(out[p]);
+        for (int p = 0; p < S; ++p) *reinterpret_cast<uint4*>(dst + p * plane) = *reinterpret_cast<uint4*>(out[p]);
     }
-    if (seg == 0) Pe[(long)(r0 + r) * Qb + blockIdx.y] = (int8_t)e;
 }
 
 // src[R][ld] with column index seg*v + b (seg < nseg, b < v)  ->
@@ -177,210 +192,360 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
 
 // ------------------------------------------------ shared MFMA helpers
 
-// 16-byte LDS fragment read: row `row` of a [64 rows][32 k] BF16 panel, the
+// 16-byte LDS fragment read: row `row` of a [rows][32 k] BF16 panel, the
 // 8 elements k = 8*seg .. 8*seg + 7.
 __device__ __forceinline__ v8s lds_frag(const char* panel, int row, int seg) {
     return *reinterpret_cast<const v8s*>(panel + lds_off(row, seg));
 }
 
-// Class sums of one KC chunk for a 32x32 wave tile: rows from panel pa,
-// columns from panel pb (each [S] slices of [64][32] at stride `sstride`).
-template <int S>
-__device__ __forceinline__ void chunk_mfma(const char* pa, const char* pb, long sstride, int wm,
-                                           int wn, int lane, v4f (&acc)[3][2][2]) {
-    // All S column slices stay in registers; row slices are loaded one at a
-    // time (slice p meets column slices q < S - p).
-    v8s b[S][2];
-    const int rr = lane & 15, sg = lane >> 4;
-#pragma unroll
-    for (int q = 0; q < S; ++q)
-#pragma unroll
-        for (int t = 0; t < 2; ++t) b[q][t] = lds_frag(pb + q * sstride, wn + 16 * t + rr, sg);
-#pragma unroll
-    for (int p = 0; p < S; ++p) {
-        v8s a[2];
-#pragma unroll
-        for (int s = 0; s < 2; ++s) a[s] = lds_frag(pa + p * sstride, wm + 16 * s + rr, sg);
-#pragma unroll
-        for (int q = 0; q < S - p; ++q)
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-#pragma unroll
-                for (int t = 0; t < 2; ++t)
-                    acc[ozaki_tier(p, q)][s][t] = mfma_bf16(a[s], b[q][t], acc[ozaki_tier(p, q)][s][t]);
-    }
+// Wait until at most N of this wave's vector-memory operations are still
+// outstanding (they retire in issue order, so every older LDS DMA load has
+// landed), then a workgroup barrier.  s_waitcnt immediate: vmcnt in bits
+// [3:0] and [15:14], expcnt [6:4] and lgkmcnt [11:8] left at their maxima.
+template <int N>
+__device__ __forceinline__ void dma_barrier() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+    __builtin_amdgcn_s_barrier();
 }
 
-// acc64 += (tier0 + tier1 + tier2) * 2^(eA(row) + eB(col) - 16); tiers = 0.
-__device__ __forceinline__ void flush(v4f (&acc)[3][2][2], v4d (&acc64)[2][2], const int (&eA)[2][4],
-                                      const int (&eB)[2]) {
+// Direct global -> LDS load of 1 KiB by one wave, 16 bytes per lane: the
+// instruction writes lane l's bytes at dst + 16*l.  `dst` must be wave-uniform.
+__device__ __forceinline__ void dma16(const uint16_t* src, char* dst) {
+    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
+// Logical 16-byte segment a lane fetches in a 16-row x 64-byte DMA block
+// (lane l lands at row l/4, physical segment l&3 of lds_off()).
+__device__ __forceinline__ int lane_seg(int lane) { return (lane & 3) ^ ((lane >> 3) & 3); }
+
+// ----------------------------------------------------------- pair kernel
+//
+// Work unit: one occupied pair (i, j) and one unit of 64 x 64 tiles.
+//   Symmetric modes: unit (A, B2) = rows a in [64A, 64A+64) x columns b in
+//   [128 B2, 128 B2 + 128), i.e. tiles (A, 2B2) and (A, 2B2+1); a tile (A, B)
+//   counts only when A <= B (A < B with the mirrored element).  Waves 0, 1
+//   form G = Bl_i^T Br_j on the two 64-column halves, waves 2, 3 the mirrored
+//   H[a][b] = sum_Q Br^Q_ja Bl^Q_ib on the same halves; H is handed to the G
+//   waves through LDS for the epilogue.
+//   Opposite-spin mode: unit (A2, B2) = 128 x 128, tiles (2A2 + x, 2B2 + y),
+//   one wave per tile, G only.
+// Stage = 16 Q of every panel row and slice, streamed by DMA into a
+// double-buffered LDS.  Panel rows (32-row blocks): symmetric: 0-1 Bl_i[A],
+// 2-5 Br_j[B], 6-7 Br_j[A], 8-11 Bl_i[B]; opposite spin: 0-3 Bl_i[A],
+// 4-7 Br_j[B].  Each wave streams 3 (2) of the blocks for every slice.
+//
+// v_mfma_f32_32x32x16_bf16 lane layout (measured on gfx950 with one-hot
+// operands): A lane l holds row l%32, k = 8(l/32)..+7; B lane l holds
+// k = 8(l/32)..+7, column l%32; D lane l, element r holds row
+// 8(r/4) + 4(l/32) + r%4, column l%32.
+
+__device__ __forceinline__ v16f mfma32_bf16(const v8s& a, const v8s& b, v16f c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0,
+                                                  0, 0);
+}
+
+// Fused denominator epilogue for one 64 x 64 tile held by one wave in the
+// 32x32 accumulator layout (2 x 2 subtiles): G (and H in the symmetric
+// modes) at row a = a0 + 32s + 8u + 4h + c, column b = b0 + 32t + l%32,
+// h = l/32.  Writes the weighted tile partials to `slot` and, when requested,
+// the amplitudes z (gradient) or t2.
+template <int MODE>
+__device__ __forceinline__ void pair_epilogue32(const PairArgs& p, int i, int j, double w, int a0, int b0,
+                                                bool mirror, int lane, const v4d (&G)[2][2][4],
+                                                const v4d (&H)[2][2][4], long slot) {
+#pragma clang fp contract(off)
+    const bool grad = p.s_part != nullptr;
+    const int h = lane >> 5, col = lane & 31;
+    const double ei = p.eol[i] + p.eor[j];
+    double e = 0.0, sm = 0.0;
+    double rowq[2][4][4], colq[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) colq[t] = 0.0;
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double sum = ((double)acc[0][s][t][r] + (double)acc[1][s][t][r]) +
-                                   (double)acc[2][s][t][r];
-                acc64[s][t][r] += sum * pow2(eA[s][r] + eB[t] - 16);
+            for (int c = 0; c < 4; ++c) rowq[s][u][c] = 0.0;
+    double* t2base = nullptr;
+    if (MODE == MODE_T2) t2base = p.Z + ((long)i * p.orr + j) * p.vl * p.vr;
+    const long zr = (long)(i - p.zi0) * p.vl, zc = (long)(j - p.zj0) * p.vr;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int b = b0 + 32 * t + col;
+        if (b >= p.vr) continue;
+        const double evb = p.evr[b];
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int a = a0 + 32 * s + 8 * u + 4 * h + c;
+                    if (a >= p.vl) continue;
+                    const double D = ei - p.evl[a] - evb;
+                    const double g = G[s][t][u][c];
+                    const double hh = (MODE == MODE_OS) ? 0.0 : H[s][t][u][c];
+                    double e1, m1, z1;
+                    pair_terms<MODE>(g, hh, D, e1, m1, z1);
+                    double q = m1, e2 = 0.0, m2 = 0.0, z2 = 0.0;
+                    e += e1;
+                    if (mirror) {
+                        pair_terms<MODE>(hh, g, D, e2, m2, z2);
+                        e += e2;
+                        q += m2;
+                    }
+                    if (grad) {
+                        sm += q;
+                        rowq[s][u][c] += q;
+                        colq[t] += q;
+                    }
+                    if (MODE == MODE_T2) {
+                        const bool ok = fabs(D) >= 1e-12;
+                        t2base[(long)a * p.vr + b] = ok ? g / D : 0.0;
+                        if (mirror) t2base[(long)b * p.vr + a] = ok ? hh / D : 0.0;
+                    } else if (p.Z) {
+                        p.Z[(zr + a) * p.ldz + zc + b] = z1;
+                        if (mirror) p.Z[(zr + b) * p.ldz + zc + a] = z2;
+                    }
+                }
+    }
+    // Deterministic reductions: fixed in-lane order, then fixed shuffle trees.
+    e = xor_sum(e, 1, 64);
+    if (lane == 0) p.e_part[slot] = w * e;
+    if (!grad) return;
+    sm = xor_sum(sm, 1, 64);
+    if (lane == 0) p.s_part[slot] = w * sm;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double v = xor_sum(rowq[s][u][c], 1, 32);
+                if (col == 0) p.vrow[slot * 64 + 32 * s + 8 * u + 4 * h + c] = w * v;
             }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const double v = xor_sum(colq[t], 32, 64);
+        if (h == 0) p.vcol[slot * 64 + 32 * t + col] = w * v;
+    }
+}
+
+template <int MODE, int S>
+__global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16_t* __restrict__ Psl,
+                                                        const int8_t* __restrict__ Pel,
+                                                        const uint16_t* __restrict__ Psr,
+                                                        const int8_t* __restrict__ Per, int Qp, int vEl,
+                                                        int vEr) {
+    constexpr bool SYM = MODE != MODE_OS;
+    constexpr int NBLK = SYM ? 12 : 8;         // 32-row panel blocks per slice and stage
+    constexpr int BPW = NBLK / 4;              // blocks streamed per wave
+    constexpr long SLICE = (long)NBLK * 1024;  // bytes per slice and stage
+    constexpr long STAGE = S * SLICE;
+    extern __shared__ __align__(16) char lds[];
+
+    const long work = xcd_work_index();
+    if (work >= (long)p.npairs * p.nunits) return;
+    const int pslot = (int)(work / p.nunits), uslot = (int)(work % p.nunits);
+    const int2 ij = p.pairs[pslot];
+    const int i = ij.x, j = ij.y;
+    const double w = p.pw[pslot];
+    const int2 un = p.units[uslot];
+    const int R0 = SYM ? un.x * 64 : un.x * 128;  // first row (left virtual index)
+    const int C0 = un.y * 128;                    // first column (right virtual index)
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+    // DMA sources: lane l of a 32-row block fetches row l/2, the logical
+    // segment that lds_off32() stores at its physical position l&1.  Rows
+    // past the operand are clamped to its last row (their outputs are
+    // discarded).
+    const long ovl = (long)p.ol * p.vl, ovr = (long)p.orr * p.vr;
+    const int lr = lane >> 1;
+    const int lseg = (lane & 1) ^ (((lr >> 2) ^ (lr >> 3)) & 1);
+    const uint16_t* rp[BPW];
+    long rplane[BPW], rstep[BPW];
+#pragma unroll
+    for (int k = 0; k < BPW; ++k) {
+        const int blk = BPW * wv + k;
+        bool left;
+        long r0;
+        if (SYM) {
+            left = blk < 2 || blk >= 8;
+            r0 = blk < 2   ? (long)i * p.vl + R0 + 32 * blk
+                 : blk < 6 ? (long)j * p.vr + C0 + 32 * (blk - 2)
+                 : blk < 8 ? (long)j * p.vr + R0 + 32 * (blk - 6)
+                           : (long)i * p.vl + C0 + 32 * (blk - 8);
+        } else {
+            left = blk < 4;
+            r0 = blk < 4 ? (long)i * p.vl + R0 + 32 * blk : (long)j * p.vr + C0 + 32 * (blk - 4);
+        }
+        const long rows = left ? ovl : ovr;
+        const long row = min(r0 + lr, rows - 1);
+        rp[k] = (left ? Psl : Psr) + row * KS + 8 * lseg;
+        rplane[k] = rows * Qp;
+        rstep[k] = rows * KS;
+    }
+    auto issue = [&](int st, int buf) {
+#pragma unroll
+        for (int sl = 0; sl < S; ++sl)
+#pragma unroll
+            for (int k = 0; k < BPW; ++k)
+                dma16(rp[k] + sl * rplane[k] + st * rstep[k], lds + buf * STAGE + sl * SLICE + (BPW * wv + k) * 1024);
+    };
+
+    // This wave's operand rows in a stage (LDS panel rows) and exponent rows.
+    int prow, pcol;          // first LDS panel row of the row / column operand
+    int arow, bcol;          // first virtual index of this wave's rows / columns
+    const int8_t *erow, *ecol;
+    int vErow, vEcol;
+    if (SYM) {
+        const int isH = wv >> 1, half = wv & 1;
+        prow = isH ? 192 : 0;
+        pcol = (isH ? 256 : 64) + 64 * half;
+        arow = R0;
+        bcol = C0 + 64 * half;
+        erow = isH ? Per + (long)j * vEr : Pel + (long)i * vEl;
+        ecol = isH ? Pel + (long)i * vEl : Per + (long)j * vEr;
+        vErow = isH ? p.orr * vEr : p.ol * vEl;
+        vEcol = isH ? p.ol * vEl : p.orr * vEr;
+    } else {
+        const int wm = wv >> 1, wn = wv & 1;
+        prow = 64 * wm;
+        pcol = 128 + 64 * wn;
+        arow = R0 + 64 * wm;
+        bcol = C0 + 64 * wn;
+        erow = Pel + (long)i * vEl;
+        ecol = Per + (long)j * vEr;
+        vErow = p.ol * vEl;
+        vEcol = p.orr * vEr;
+    }
+    const int h = lane >> 5, fr = lane & 31;
+
+    v16f acc[3][2][2];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int t = 0; t < 2; ++t) acc[c][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
-}
-
-// ----------------------------------------------------------- pair kernel
-
-template <int MODE, int S>
-__global__ __launch_bounds__(MODE == MODE_OS ? 256 : 512) void pair_bf16_kernel(
-    PairArgs p, const uint16_t* __restrict__ Psl, const int8_t* __restrict__ Pel,
-    const uint16_t* __restrict__ Psr, const int8_t* __restrict__ Per, int Qp, int Qb) {
-    constexpr bool SYM = MODE != MODE_OS;
-    constexpr int NP = SYM ? 4 : 2;
-    constexpr int NT = SYM ? 512 : 256;
-    constexpr long PSTRIDE = (long)S * 64 * LROW;  // bytes per panel
-    extern __shared__ __align__(16) char lds[];
-    __shared__ double red_row[2][64], red_col[2][64], red_e[4], red_s[4];
-
-    const long work = xcd_work_index();
-    if (work >= p.nwork) return;
-    const int pslot = (int)(work / p.ntiles), tslot = (int)(work % p.ntiles);
-    const int2 ij = p.pairs[pslot];
-    const int i = ij.x, j = ij.y;
-    const double w = p.pw[pslot];
-    const int2 tb = p.tiles[tslot];
-    const int A0 = tb.x * 64, B0 = tb.y * 64;
-    const bool mirror = SYM && (tb.x != tb.y);
-
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wq = wave & 3, isH = wave >> 2;
-    const int wm = (wq >> 1) * 32, wn = (wq & 1) * 32;
-
-    // Panel 0: Bl_i[A] (G rows), 1: Br_j[B] (G cols), 2: Br_j[A] (H rows),
-    // 3: Bl_i[B] (H cols).  Selected with branches, not an indexed array,
-    // so nothing lands in scratch.
-    const uint16_t* pb0 = Psl + ((long)i * p.vl + A0) * Qp;
-    const uint16_t* pb1 = Psr + ((long)j * p.vr + B0) * Qp;
-    const uint16_t* pb2 = Psr + ((long)j * p.vr + A0) * Qp;
-    const uint16_t* pb3 = Psl + ((long)i * p.vl + B0) * Qp;
-    const long sl_l = (long)p.ol * p.vl * Qp, sl_r = (long)p.orr * p.vr * Qp;
-    const int lim0 = p.vl - A0, lim1 = p.vr - B0, lim2 = p.vr - A0, lim3 = p.vl - B0;
-
-    constexpr int NSEG = NP * S * 256;  // 16-byte segments per chunk
-    constexpr int PER = (NSEG + NT - 1) / NT;
-    uint4 reg[PER];
-    auto fetch = [&](int q0) {
+            for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int g = tid + u * NT;
-            uint4 val = make_uint4(0, 0, 0, 0);
-            if (g < NSEG) {
-                const int P = g / (S * 256), sl = (g / 256) % S, row = (g & 255) >> 2, sg = g & 3;
-                const uint16_t* base = P == 0 ? pb0 : P == 1 ? pb1 : P == 2 ? pb2 : pb3;
-                const long pst = (P == 0 || P == 3) ? sl_l : sl_r;
-                const int lim = P == 0 ? lim0 : P == 1 ? lim1 : P == 2 ? lim2 : lim3;
-                if (row < lim)
-                    val = *reinterpret_cast<const uint4*>(base + sl * pst + (long)row * Qp + q0 + 8 * sg);
-            }
-            reg[u] = val;
-        }
-    };
-    auto stash = [&]() {
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int g = tid + u * NT;
-            if (g < NSEG) {
-                const int P = g / (S * 256), sl = (g / 256) % S, row = (g & 255) >> 2, sg = g & 3;
-                *reinterpret_cast<uint4*>(lds + P * PSTRIDE + (long)sl * 64 * LROW + lds_off(row, sg)) =
-                    reg[u];
-            }
-        }
-    };
-
-    v4f acc[3][2][2];
-    v4d acc64[2][2];
+                for (int r = 0; r < 16; ++r) acc[c][s][t][r] = 0.f;
+    v4d acc64[2][2][4];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            acc64[s][t] = v4d{0.0, 0.0, 0.0, 0.0};
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) acc[c][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
-        }
-    // This wave's operand panels and exponent rows.
-    const char* pa = lds + (isH ? 2 : 0) * PSTRIDE;
-    const char* pb = lds + (isH ? 3 : 1) * PSTRIDE;
-    const int8_t* exA = isH ? Per + ((long)j * p.vr) * Qb : Pel + ((long)i * p.vl) * Qb;
-    const int8_t* exB = isH ? Pel + ((long)i * p.vl) * Qb : Per + ((long)j * p.vr) * Qb;
-    const int limA = isH ? p.vr : p.vl, limB = isH ? p.vl : p.vr;
+            for (int u = 0; u < 4; ++u) acc64[s][t][u] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    // Register prefetch of the next chunk overlaps the loads with the MFMAs.
-    constexpr bool PREFETCH = true;
-    const int nchunk = Qp / KC;
-    if (PREFETCH) fetch(0);
-    for (int ch = 0; ch < nchunk; ++ch) {
-        __syncthreads();
-        if (!PREFETCH) fetch(ch * KC);
-        stash();
-        __syncthreads();
-        if (PREFETCH && ch + 1 < nchunk) fetch((ch + 1) * KC);
-        chunk_mfma<S>(pa, pb, (long)64 * LROW, wm, wn, lane, acc);
-        if ((ch + 1) % (QB / KC) == 0) {
-            const int qb = ch / (QB / KC);
-            int eA[2][4], eB[2];
+    const int nst = Qp / KS;
+    constexpr int SPB = QB / KS;  // stages per exponent block (even: buffer = stage parity)
+    issue(0, 0);
+    dma_barrier<0>();
+    for (int qb = 0; qb < Qp / QB; ++qb) {
+        // Exponents of this Q block, fetched early and without branches
+        // (rows past the operand read a clamped row; those outputs are
+        // discarded).  Row exponents come four rows to a dword: the tables
+        // are [Qb][o][vE] with vE a multiple of 64.
+        int erw[2][4], ecl[2];
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+        for (int s = 0; s < 2; ++s)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int a = A0 + wm + 16 * s + bf16_row(lane, r);
-                    eA[s][r] = a < limA ? exA[(long)a * Qb + qb] : 0;
-                }
+            for (int u = 0; u < 4; ++u)
+                erw[s][u] = *reinterpret_cast<const int*>(erow + (long)qb * vErow + arow + 32 * s + 8 * u + 4 * h);
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int b = B0 + wn + 16 * t + (lane & 15);
-                eB[t] = b < limB ? exB[(long)b * Qb + qb] : 0;
+        for (int t = 0; t < 2; ++t) ecl[t] = ecol[(long)qb * vEcol + bcol + 32 * t + fr];
+#pragma unroll
+        for (int c8 = 0; c8 < SPB; ++c8) {
+            const int st = qb * SPB + c8;
+            const int buf = c8 & 1;
+            if (st + 1 < nst) issue(st + 1, buf ^ 1);
+            const char* base = lds + buf * STAGE;
+            v8s bf[S][2];
+#pragma unroll
+            for (int q = 0; q < S; ++q)
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    bf[q][t] = *reinterpret_cast<const v8s*>(base + q * SLICE + lds_off32(pcol + 32 * t + fr, h));
+#pragma unroll
+            for (int pp = 0; pp < S; ++pp) {
+                v8s af[2];
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    af[s] = *reinterpret_cast<const v8s*>(base + pp * SLICE + lds_off32(prow + 32 * s + fr, h));
+#pragma unroll
+                for (int q = 0; q < S - pp; ++q)
+#pragma unroll
+                    for (int s = 0; s < 2; ++s)
+#pragma unroll
+                        for (int t = 0; t < 2; ++t)
+                            acc[ozaki_tier(pp, q)][s][t] = mfma32_bf16(af[s], bf[q][t], acc[ozaki_tier(pp, q)][s][t]);
             }
-            flush(acc, acc64, eA, eB);
+            dma_barrier<0>();
         }
+        // FP64 flush of the block: (tier0 + tier1) + tier2, scaled by
+        // 2^(e_row + e_col - 16).
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int er = (int)(signed char)(erw[s][r >> 2] >> (8 * (r & 3)));
+                    const double sum =
+                        ((double)acc[0][s][t][r] + (double)acc[1][s][t][r]) + (double)acc[2][s][t][r];
+                    acc64[s][t][r >> 2][r & 3] += sum * pow2(er + ecl[t] - 16);
+                    acc[0][s][t][r] = acc[1][s][t][r] = acc[2][s][t][r] = 0.f;
+                }
     }
 
-    v4d accH[2][2];
+    v4d accH[2][2][4];
     if (SYM) {
-        // H waves hand their tile to the G waves through LDS.
-        __syncthreads();
-        double* Hs = reinterpret_cast<double*>(lds);  // [64][65]
+        // H waves hand their tile to the G waves through LDS (same lane map).
+        double* Hs = reinterpret_cast<double*>(lds);
+        const int isH = wv >> 1, half = wv & 1;
         if (isH) {
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
                 for (int t = 0; t < 2; ++t)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        Hs[(wm + 16 * s + bf16_row(lane, r)) * 65 + wn + 16 * t + (lane & 15)] =
-                            acc64[s][t][r];
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            Hs[((((half * 2 + s) * 2 + t) * 4 + u) * 4 + c) * 64 + lane] = acc64[s][t][u][c];
         }
         __syncthreads();
-        if (!isH) {
+        if (isH) return;
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+        for (int s = 0; s < 2; ++s)
 #pragma unroll
-                for (int t = 0; t < 2; ++t)
+            for (int t = 0; t < 2; ++t)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        accH[s][t][r] =
-                            Hs[(wm + 16 * s + bf16_row(lane, r)) * 65 + wn + 16 * t + (lane & 15)];
-        }
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        accH[s][t][u][c] = Hs[((((half * 2 + s) * 2 + t) * 4 + u) * 4 + c) * 64 + lane];
+        const int A = un.x, B = 2 * un.y + half;
+        if (B >= p.ntB || A > B) return;
+        const long tslot = (long)A * p.ntA - (long)A * (A - 1) / 2 + (B - A);
+        pair_epilogue32<MODE>(p, i, j, w, 64 * A, 64 * B, A < B, lane, acc64, accH,
+                              (long)pslot * p.ntiles + tslot);
     } else {
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int t = 0; t < 2; ++t) accH[s][t] = v4d{0.0, 0.0, 0.0, 0.0};
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) accH[s][t][u] = v4d{0.0, 0.0, 0.0, 0.0};
+        const int A = 2 * un.x + (wv >> 1), B = 2 * un.y + (wv & 1);
+        if (A >= p.ntA || B >= p.ntB) return;
+        pair_epilogue32<MODE>(p, i, j, w, 64 * A, 64 * B, false, lane, acc64, accH,
+                              (long)pslot * p.ntiles + (long)A * p.ntB + B);
     }
-    pair_epilogue<MODE, 1>(p, i, j, w, A0, B0, mirror, lane, wq, wm, wn, acc64, accH, red_row,
-                           red_col, red_e, red_s, work, !isH);
 }
 
 // ----------------------------------------------------------- GEMM (NT)
@@ -393,27 +558,11 @@ struct SlicedMat {
     int rows;
 };
 
-// Wait for this wave's outstanding vector-memory operations (the LDS DMA
-// loads), then a workgroup barrier.
-__device__ __forceinline__ void dma_barrier() {
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-    __syncthreads();
-}
-
-// Direct global -> LDS load of 16 BF16 panel rows x 64 bytes by one wave.
-// The instruction writes lane l's 16 bytes at dst + 16*l, i.e. panel row
-// l/4 of the 16, physical segment l&3; lane_seg() is the logical segment
-// that lds_off() places there.  `dst` must be wave-uniform.
-__device__ __forceinline__ int lane_seg(int lane) { return (lane & 3) ^ ((lane >> 3) & 3); }
-
-__device__ __forceinline__ void dma16(const uint16_t* src, char* dst) {
-    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
-}
-
 // C[m][n] += sum_k A[m][k] X[n][k] for the gradient contractions; K is a
 // multiple of GB.  128 x 128 workgroup tile, four waves of 64 x 64, one
-// 32-wide k step per stage; stages are loaded straight into a double-buffered
-// LDS by the DMA path (one barrier per stage, no staging registers).  The
+// 32-wide k step per stage; stages are loaded straight into a triple-buffered
+// LDS by the DMA path, two stages ahead (one barrier per stage, no staging
+// registers).  The
 // gradient needs ~1e-7 relative accuracy, so products go to two FP32 tiers:
 // c = 0 (exact integers) and c >= 1 (rounding near 2^-31 of the block
 // scale); both are flushed to FP64 with the block exponents once per GB-wide
@@ -423,6 +572,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
                                                            long ldc) {
     constexpr long PANEL = (long)GT * LROW;   // bytes of one slice panel (128 rows x 32 k)
     constexpr long STAGE = 2L * S * PANEL;    // both operands, all slices
+    constexpr int NBUF = 3;
     constexpr int PER = 2 * S * (GT / 16) / 4; // 16-row DMA loads per wave and stage
     extern __shared__ __align__(16) char lds[];
     int mt, ntile;
@@ -462,28 +612,33 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
     const int rr = lane & 15, fs = lane >> 4;
     const int nkb = K / GB;
     constexpr int CPB = GB / KC;  // chunks per exponent block (even: buffer = chunk parity)
+    const int nchunk = nkb * CPB;  // >= 4
     issue(0, 0);
-    dma_barrier();
+    issue(KC, 1);
+    dma_barrier<PER>();
     for (int kb = 0; kb < nkb; ++kb) {
-        // Exponents of this block, fetched early; used by the flush.
+        // Exponents of this block, fetched early and without branches (rows
+        // past the matrix read the last row's: their outputs are discarded),
+        // so the loads issue back to back and their wait lands at the flush.
         int eA[4][4], eB[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const int n = n0 + wn + 16 * t + rr;
-            eB[t] = n < X.rows ? X.e[(long)n * X.lde + kb] : 0;
+            const int n = min(n0 + wn + 16 * t + rr, X.rows - 1);
+            eB[t] = X.e[(long)n * X.lde + kb];
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm + 16 * s + 4 * fs + r;
-                eA[s][r] = m < A.rows ? A.e[(long)m * A.lde + kb] : 0;
+                const int m = min(m0 + wm + 16 * s + 4 * fs + r, A.rows - 1);
+                eA[s][r] = A.e[(long)m * A.lde + kb];
             }
 #pragma unroll
         for (int c = 0; c < CPB; ++c) {
-            const int buf = c & 1;
-            if (kb * CPB + c + 1 < nkb * CPB) issue((kb * CPB + c + 1) * KC, buf ^ 1);
-            const char* pa = lds + buf * STAGE;
+            const int ch = kb * CPB + c;
+            const bool ahead = ch + 2 < nchunk;
+            if (ahead) issue((ch + 2) * KC, (ch + 2) % NBUF);
+            const char* pa = lds + (ch % NBUF) * STAGE;
             const char* pb = pa + S * PANEL;
             v8s b[S][4];
 #pragma unroll
@@ -503,7 +658,9 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
                         for (int t = 0; t < 4; ++t)
                             acc[p + q ? 1 : 0][s][t] = mfma_bf16(a[s], b[q][t], acc[p + q ? 1 : 0][s][t]);
             }
-            dma_barrier();
+            // Stage ch+1 must have landed; stage ch+2 may stay in flight.
+            if (ahead) dma_barrier<PER>();
+            else dma_barrier<0>();
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -558,15 +715,16 @@ int by_gslices(int S, Args&&... args) {
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-// Dynamic LDS of the gradient GEMM: two stages of both operands' slices.
-constexpr size_t gemm_lds(int S) { return (size_t)2 * 2 * S * GT * LROW; }
+// Dynamic LDS of the gradient GEMM: three stages of both operands' slices.
+constexpr size_t gemm_lds(int S) { return (size_t)3 * 2 * S * GT * LROW; }
 
 template <int S>
 struct SlicePair {
-    static int run(mshqc_hip_ctx* ctx, const double* B, int Q, int ov, int Qp, int Qb, uint16_t* Ps,
+    static int run(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v, int vE, int Qp, uint16_t* Ps,
                    int8_t* Pe) {
-        dim3 grid(ceil_div(ov, 64), Qb);
-        slice_pair_kernel<S><<<grid, 256, 0, ctx->stream>>>(B, Q, ov, Qp, Qb, Ps, Pe);
+        const int ov = o * v;
+        dim3 grid(ceil_div(ov, 64), Qp / QB);
+        slice_pair_kernel<S><<<grid, 256, 0, ctx->stream>>>(B, Q, ov, v, vE, o, Qp, Ps, Pe);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -594,25 +752,33 @@ struct SliceCols {
     }
 };
 
+// Dynamic LDS of the pair kernel: two stages of every panel row and slice;
+// the symmetric kernel reuses it for the two 64 x 64 FP64 H tiles (64 KiB).
+constexpr size_t pair_lds(int S, bool sym) {
+    return sym ? std::max((size_t)2 * S * 12 * 1024, (size_t)2 * 64 * 64 * 8) : (size_t)2 * S * 8 * 1024;
+}
+
 template <int S>
 struct PairLaunch {
     static int run(mshqc_hip_ctx* ctx, int mode, const PairArgs& a, const Bf16Operands& o) {
-        const long nwg = ((a.nwork + 7) / 8) * 8;
+        const long nwork = (long)a.npairs * a.nunits;
+        const long nwg = ((nwork + 7) / 8) * 8;
+        if (nwg > 0x7fffffffL) return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 pair kernel: grid too large");
         dim3 grid((unsigned)nwg);
-        // The symmetric kernel reuses its panels for the 64 x 65 FP64 H tile.
-        const size_t lds_sym = std::max((size_t)4 * S * 64 * LROW, (size_t)64 * 65 * 8);
-        const size_t lds_os = (size_t)2 * S * 64 * LROW;
         const uint16_t* sr = o.sr ? o.sr : o.sl;
         const int8_t* xr = o.sr ? o.xr : o.xl;
         switch (mode) {
             case MODE_RMP2:
-                pair_bf16_kernel<MODE_RMP2, S><<<grid, 512, lds_sym, ctx->stream>>>(a, o.sl, o.xl, sr, xr, o.Qp, o.qblocks);
+                pair_bf16_kernel<MODE_RMP2, S><<<grid, 256, pair_lds(S, true), ctx->stream>>>(
+                    a, o.sl, o.xl, sr, xr, o.Qp, o.vEl, o.vEr);
                 break;
             case MODE_SS:
-                pair_bf16_kernel<MODE_SS, S><<<grid, 512, lds_sym, ctx->stream>>>(a, o.sl, o.xl, sr, xr, o.Qp, o.qblocks);
+                pair_bf16_kernel<MODE_SS, S><<<grid, 256, pair_lds(S, true), ctx->stream>>>(
+                    a, o.sl, o.xl, sr, xr, o.Qp, o.vEl, o.vEr);
                 break;
             case MODE_OS:
-                pair_bf16_kernel<MODE_OS, S><<<grid, 256, lds_os, ctx->stream>>>(a, o.sl, o.xl, sr, xr, o.Qp, o.qblocks);
+                pair_bf16_kernel<MODE_OS, S><<<grid, 256, pair_lds(S, false), ctx->stream>>>(
+                    a, o.sl, o.xl, sr, xr, o.Qp, o.vEl, o.vEr);
                 break;
             default:
                 return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 pair kernel: unsupported mode %d", mode);
@@ -638,13 +804,12 @@ template <int S>
 struct AllowLds {
     static int run(mshqc_hip_ctx* ctx) {
         // Dynamic LDS above 64 KiB must be opted into per kernel.
-        const int big = std::max(4 * S * 64 * LROW, 64 * 65 * 8);
         hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_RMP2, S>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, big);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, true));
         hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_SS, S>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, big);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, true));
         hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_OS, S>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, big / 2);
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, false));
         hipGetLastError();
         (void)ctx;
         return 0;
@@ -676,25 +841,29 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     o.Qp = round_up(Q, QB);
     o.qblocks = o.Qp / QB;
     const long ovl = (long)ol * vl, ovr = Br ? (long)orr * vr : 0;
-    // Pair-kernel slices.
+    // Pair-kernel slices [S][Qp/16][o*v][16] and exponent tables
+    // [Qp/128][o][vE], vE = v rounded up to 128 (zero padding).
+    o.vEl = round_up(vl, 128);
+    o.vEr = Br ? round_up(vr, 128) : o.vEl;
     const size_t bl = (size_t)S * ovl * o.Qp * 2, br = (size_t)S * ovr * o.Qp * 2;
-    const size_t el = (size_t)ovl * o.qblocks, er = (size_t)ovr * o.qblocks;
+    const size_t el = (size_t)o.qblocks * ol * o.vEl, er = Br ? (size_t)o.qblocks * orr * o.vEr : 0;
     char* w = static_cast<char*>(workspace(ctx, WS_SLICE, bl + br + el + er + 1024));
     if (!w) return MSHQC_HIP_ERR_ALLOC;
     o.sl = reinterpret_cast<uint16_t*>(w);
     o.sr = Br ? reinterpret_cast<uint16_t*>(w + bl) : nullptr;
     o.xl = reinterpret_cast<int8_t*>(w + bl + br);
     o.xr = Br ? reinterpret_cast<int8_t*>(w + bl + br + el) : nullptr;
-    int rc = by_slices<SlicePair>(S, ctx, Bl, Q, (int)ovl, o.Qp, o.qblocks, const_cast<uint16_t*>(o.sl),
+    HIP_TRY(ctx, hipMemsetAsync(w + bl + br, 0, el + er, ctx->stream));
+    int rc = by_slices<SlicePair>(S, ctx, Bl, Q, ol, vl, o.vEl, o.Qp, const_cast<uint16_t*>(o.sl),
                                   const_cast<int8_t*>(o.xl));
     if (rc) return rc;
     if (Br) {
-        rc = by_slices<SlicePair>(S, ctx, Br, Q, (int)ovr, o.Qp, o.qblocks, const_cast<uint16_t*>(o.sr),
+        rc = by_slices<SlicePair>(S, ctx, Br, Q, orr, vr, o.vEr, o.Qp, const_cast<uint16_t*>(o.sr),
                                   const_cast<int8_t*>(o.xr));
         if (rc) return rc;
     }
     if (grad) {
-        // GEMM-operand slices of B: rows q, columns (i, a) blocked by 64 along a.
+        // GEMM-operand slices of B: rows q, columns (i, a) blocked by 128 along a.
         o.vlp = round_up(vl, GB);
         o.vrp = round_up(vr, GB);
         const int SG = o.gslices;

@@ -113,6 +113,17 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
     for (int A = 0; A < ntA; ++A)
         for (int B = sym ? A : 0; B < ntB; ++B) tiles.push_back(make_int2(A, B));
     const int ntiles = (int)tiles.size();
+    // Work units of the split-BF16 kernel: symmetric modes (A, B2) = tiles
+    // (A, 2B2), (A, 2B2+1) with A <= 2B2+1; opposite spin (A2, B2) = 2 x 2 tiles.
+    std::vector<int2> units;
+    if (sym) {
+        for (int A = 0; A < ntA; ++A)
+            for (int B2 = 0; B2 < (ntA + 1) / 2; ++B2)
+                if (A <= 2 * B2 + 1) units.push_back(make_int2(A, B2));
+    } else {
+        for (int A2 = 0; A2 < (ntA + 1) / 2; ++A2)
+            for (int B2 = 0; B2 < (ntB + 1) / 2; ++B2) units.push_back(make_int2(A2, B2));
+    }
 
     // Pair lists: one segment per launch.
     struct Seg {
@@ -168,21 +179,24 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
     int maxseg = 0;
     for (auto& s : segs) maxseg = std::max(maxseg, s.n);
 
-    // Upload tiles | pairs | weights, 16-byte aligned pieces.
-    const size_t tb = sizeof(int2) * tiles.size(), pb = sizeof(int2) * pairs.size(),
-                 wb = sizeof(double) * wts.size();
-    const size_t o_pairs = (tb + 255) / 256 * 256, o_w = o_pairs + (pb + 255) / 256 * 256;
+    // Upload tiles | units | pairs | weights, 256-byte aligned pieces.
+    const size_t tb = sizeof(int2) * tiles.size(), ub = sizeof(int2) * units.size(),
+                 pb = sizeof(int2) * pairs.size(), wb = sizeof(double) * wts.size();
+    const size_t o_units = (tb + 255) / 256 * 256;
+    const size_t o_pairs = o_units + (ub + 255) / 256 * 256, o_w = o_pairs + (pb + 255) / 256 * 256;
     const size_t o_spair = o_w + (wb + 255) / 256 * 256;
     const size_t misc_bytes = o_spair + sizeof(double) * (size_t)maxseg + 256;
     char* misc = static_cast<char*>(workspace(ctx, WS_MISC, misc_bytes));
     if (!misc) return MSHQC_HIP_ERR_ALLOC;
     std::vector<char> host(o_spair, 0);
     std::memcpy(host.data(), tiles.data(), tb);
+    std::memcpy(host.data() + o_units, units.data(), ub);
     std::memcpy(host.data() + o_pairs, pairs.data(), pb);
     std::memcpy(host.data() + o_w, wts.data(), wb);
     HIP_TRY(ctx, hipMemcpyAsync(misc, host.data(), o_spair, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     const int2* d_tiles = reinterpret_cast<const int2*>(misc);
+    const int2* d_units = reinterpret_cast<const int2*>(misc + o_units);
     const int2* d_pairs = reinterpret_cast<const int2*>(misc + o_pairs);
     const double* d_w = reinterpret_cast<const double*>(misc + o_w);
     double* d_spair = reinterpret_cast<double*>(misc + o_spair);
@@ -233,6 +247,10 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
         a.npairs = sg.n;
         a.tiles = d_tiles;
         a.ntiles = ntiles;
+        a.units = d_units;
+        a.nunits = (int)units.size();
+        a.ntA = ntA;
+        a.ntB = ntB;
         a.nwork = (long)sg.n * ntiles;
         a.e_part = part;
         if (pr.mode == MODE_T2) {

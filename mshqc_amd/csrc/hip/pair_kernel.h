@@ -30,6 +30,11 @@ struct PairArgs {
     const int2* tiles;                    // (A, B) tile indices
     int ntiles;
     long nwork;                           // npairs * ntiles
+    // Work units of the split-BF16 kernel (groups of 64 x 64 tiles; the
+    // partials above stay per tile).  ntA, ntB: 64-wide tiles along vl, vr.
+    const int2* units;
+    int nunits;
+    int ntA, ntB;
     double* e_part;                       // [nwork] weighted energy partials
     // Gradient outputs (null for energy only).
     double* Z;       // amplitude block, row (i - zi0) * vl + a, col (j - zj0) * vr + b

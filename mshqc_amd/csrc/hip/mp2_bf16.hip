@@ -198,15 +198,20 @@ __device__ __forceinline__ v8s lds_frag(const char* panel, int row, int seg) {
     return *reinterpret_cast<const v8s*>(panel + lds_off(row, seg));
 }
 
-// Wait until at most N of this wave's vector-memory operations are still
-// outstanding (they retire in issue order, so every older LDS DMA load has
-// landed), then a workgroup barrier.  s_waitcnt immediate: vmcnt in bits
-// [3:0] and [15:14], expcnt [6:4] and lgkmcnt [11:8] left at their maxima.
+// Stage barrier of the DMA pipelines: wait until at most N of this wave's
+// vector-memory operations are outstanding (they retire in issue order, so
+// every older LDS DMA load has landed) and all of its LDS reads have
+// returned, then a workgroup barrier.  The empty asm statements keep the
+// compiler from moving LDS or global accesses across it.  s_waitcnt
+// immediate: vmcnt in bits [3:0] and [15:14], expcnt [6:4] (left at its
+// maximum), lgkmcnt [11:8] = 0.
 template <int N>
 __device__ __forceinline__ void dma_barrier() {
     static_assert(N >= 0 && N < 64, "vmcnt range");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0070);
     __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
 }
 
 // Direct global -> LDS load of 1 KiB by one wave, 16 bytes per lane: the
@@ -271,6 +276,7 @@ __device__ __forceinline__ void pair_epilogue32(const PairArgs& p, int i, int j,
     double* t2base = nullptr;
     if (MODE == MODE_T2) t2base = p.Z + ((long)i * p.orr + j) * p.vl * p.vr;
     const long zr = (long)(i - p.zi0) * p.vl, zc = (long)(j - p.zj0) * p.vr;
+    const long zr2 = (long)(j - p.zi0) * p.vl, zc2 = (long)(i - p.zj0) * p.vr;
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int b = b0 + 32 * t + col;
@@ -308,6 +314,10 @@ __device__ __forceinline__ void pair_epilogue32(const PairArgs& p, int i, int j,
                     } else if (p.Z) {
                         p.Z[(zr + a) * p.ldz + zc + b] = z1;
                         if (mirror) p.Z[(zr + b) * p.ldz + zc + a] = z2;
+                        if (p.ztrans && i != j) {
+                            p.Z[(zr2 + b) * p.ldz + zc2 + a] = z1;
+                            if (mirror) p.Z[(zr2 + a) * p.ldz + zc2 + b] = z2;
+                        }
                     }
                 }
     }
@@ -388,12 +398,12 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
         rplane[k] = rows * Qp;
         rstep[k] = rows * KS;
     }
-    auto issue = [&](int st, int buf) {
+    // One slice of a stage; a stage's loads are spread over its MFMA groups
+    // (a burst of them stalls instruction issue behind the address unit).
+    auto issue_slice = [&](int st, int buf, int sl) {
 #pragma unroll
-        for (int sl = 0; sl < S; ++sl)
-#pragma unroll
-            for (int k = 0; k < BPW; ++k)
-                dma16(rp[k] + sl * rplane[k] + st * rstep[k], lds + buf * STAGE + sl * SLICE + (BPW * wv + k) * 1024);
+        for (int k = 0; k < BPW; ++k)
+            dma16(rp[k] + sl * rplane[k] + st * rstep[k], lds + buf * STAGE + sl * SLICE + (BPW * wv + k) * 1024);
     };
 
     // This wave's operand rows in a stage (LDS panel rows) and exponent rows.
@@ -443,7 +453,8 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 
     const int nst = Qp / KS;
     constexpr int SPB = QB / KS;  // stages per exponent block (even: buffer = stage parity)
-    issue(0, 0);
+#pragma unroll
+    for (int sl = 0; sl < S; ++sl) issue_slice(0, 0, sl);
     dma_barrier<0>();
     for (int qb = 0; qb < Qp / QB; ++qb) {
         // Exponents of this Q block, fetched early and without branches
@@ -462,7 +473,7 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
         for (int c8 = 0; c8 < SPB; ++c8) {
             const int st = qb * SPB + c8;
             const int buf = c8 & 1;
-            if (st + 1 < nst) issue(st + 1, buf ^ 1);
+            const bool ahead = st + 1 < nst;
             const char* base = lds + buf * STAGE;
             v8s bf[S][2];
 #pragma unroll
@@ -472,6 +483,7 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
                     bf[q][t] = *reinterpret_cast<const v8s*>(base + q * SLICE + lds_off32(pcol + 32 * t + fr, h));
 #pragma unroll
             for (int pp = 0; pp < S; ++pp) {
+                if (ahead) issue_slice(st + 1, buf ^ 1, pp);
                 v8s af[2];
 #pragma unroll
                 for (int s = 0; s < 2; ++s)
@@ -588,9 +600,11 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
     const int lrow = lane >> 2;
     const long laneA = (long)lrow * A.ld + 8 * lane_seg(lane);
     const long laneX = (long)lrow * X.ld + 8 * lane_seg(lane);
-    auto issue = [&](int k0, int buf) {
+    // Loads u in [4g, 4g+4) of a stage; a stage's loads are spread over its
+    // S MFMA groups (PER = 4S).
+    auto issue_part = [&](int k0, int buf, int g) {
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
+        for (int u = 4 * g; u < 4 * g + 4; ++u) {
             const int id = wv + 4 * u;
             const int P = id / (8 * S), sl = (id / 8) % S, rb = id % 8;
             const SlicedMat& M = P ? X : A;
@@ -613,8 +627,11 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
     const int nkb = K / GB;
     constexpr int CPB = GB / KC;  // chunks per exponent block (even: buffer = chunk parity)
     const int nchunk = nkb * CPB;  // >= 4
-    issue(0, 0);
-    issue(KC, 1);
+    static_assert(PER == 4 * S, "four loads per MFMA group");
+#pragma unroll
+    for (int g = 0; g < S; ++g) issue_part(0, 0, g);
+#pragma unroll
+    for (int g = 0; g < S; ++g) issue_part(KC, 1, g);
     dma_barrier<PER>();
     for (int kb = 0; kb < nkb; ++kb) {
         // Exponents of this block, fetched early and without branches (rows
@@ -637,7 +654,6 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
         for (int c = 0; c < CPB; ++c) {
             const int ch = kb * CPB + c;
             const bool ahead = ch + 2 < nchunk;
-            if (ahead) issue((ch + 2) * KC, (ch + 2) % NBUF);
             const char* pa = lds + (ch % NBUF) * STAGE;
             const char* pb = pa + S * PANEL;
             v8s b[S][4];
@@ -647,6 +663,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
                 for (int t = 0; t < 4; ++t) b[q][t] = lds_frag(pb + q * PANEL, wn + 16 * t + rr, fs);
 #pragma unroll
             for (int p = 0; p < S; ++p) {
+                if (ahead) issue_part((ch + 2) * KC, (ch + 2) % NBUF, p);
                 v8s a[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) a[s] = lds_frag(pa + p * PANEL, wm + 16 * s + rr, fs);

@@ -164,10 +164,14 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
             for (int J0 = sym ? I0 : 0; J0 < pr.orr; J0 += bo_r) {
                 const int nJ = std::min(bo_r, pr.orr - J0);
                 Seg sg{(int)pairs.size(), 0, I0, nI, J0, nJ};
+                // A diagonal block of a symmetric mode takes each unordered
+                // pair once (weight 2 off the diagonal); the kernel writes
+                // both its Z entries.
                 for (int i = I0; i < I0 + nI; ++i)
                     for (int j = J0; j < J0 + nJ; ++j) {
+                        if (sym && I0 == J0 && j < i) continue;
                         pairs.push_back(make_int2(i, j));
-                        wts.push_back(sym && I0 != J0 ? 2.0 : 1.0);
+                        wts.push_back(sym && i != j ? 2.0 : 1.0);
                     }
                 sg.n = (int)pairs.size() - sg.off;
                 segs.push_back(sg);
@@ -263,6 +267,7 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
             a.ldz = ldz;
             a.zi0 = sg.I0;
             a.zj0 = sg.J0;
+            a.ztrans = (sym && sg.I0 == sg.J0) ? 1 : 0;
         }
         int rc;
         {

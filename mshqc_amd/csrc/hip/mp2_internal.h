@@ -82,6 +82,11 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& p, int i, int j, d
                     const long r1 = (long)(i - p.zi0) * p.vl, c1 = (long)(j - p.zj0) * p.vr;
                     p.Z[(r1 + a) * p.ldz + c1 + b] = z1;
                     if (mirror) p.Z[(r1 + b) * p.ldz + c1 + a] = z2;
+                    if (p.ztrans && i != j) {
+                        const long r2 = (long)(j - p.zi0) * p.vl, c2 = (long)(i - p.zj0) * p.vr;
+                        p.Z[(r2 + b) * p.ldz + c2 + a] = z1;
+                        if (mirror) p.Z[(r2 + a) * p.ldz + c2 + b] = z2;
+                    }
                 }
             }
     // Deterministic tile reductions: fixed shuffle trees, then fixed-order

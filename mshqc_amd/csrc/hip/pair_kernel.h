@@ -40,6 +40,8 @@ struct PairArgs {
     double* Z;       // amplitude block, row (i - zi0) * vl + a, col (j - zj0) * vr + b
     long ldz;
     int zi0, zj0;
+    int ztrans;      // diagonal block of a symmetric mode: pairs i < j also
+                     // write the (j, i) entries, z^{ji}_{ba} = z^{ij}_{ab}
     double* s_part;  // [nwork]      weighted sum of m over the tile
     double* vrow;    // [nwork][64]  weighted row sums of m (left virtual index)
     double* vcol;    // [nwork][64]  weighted column sums of m (right virtual index)

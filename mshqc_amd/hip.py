@@ -74,10 +74,12 @@ _LOCK = threading.Lock()
 
 
 def lib() -> ctypes.CDLL:
+    """The process-wide library: MSHQC_HIP_LIBRARY names another build of it
+    (same ABI), else the in-tree libmshqc_hip.so."""
     global _LIB
     with _LOCK:
         if _LIB is None:
-            _LIB = load_library()
+            _LIB = load_library(os.environ.get("MSHQC_HIP_LIBRARY", LIB_PATH))
     return _LIB
 
 

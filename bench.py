@@ -84,6 +84,20 @@ def cpu_baseline(B, eo, ev, npairs, seed=7):
     }
 
 
+def measured_traffic(kernel: str, workload: str, precision: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the calibration kernel), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}_{precision}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,12 +134,14 @@ def main():
     ctx = hip.context(0)
     W = rmp2_work(Q, o, v)
     if args.copy_calibration:
+        # A plain streaming kernel of known size with a name of its own
+        # (sqrt): it reads and writes nel * 8 bytes once each.
         nel = args.copy_calibration * (1 << 20) // 8
-        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        src = torch.full((nel,), 2.0, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.sqrt(src, out=dst)
         torch.cuda.synchronize()
-        log(f"[bench] calibration copy: {nel * 8} bytes read, {nel * 8} bytes written")
+        log(f"[bench] calibration sqrt: {nel * 8} bytes read, {nel * 8} bytes written")
         del src, dst
 
     results = {}
@@ -198,7 +214,8 @@ def main():
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": (pair_tflops * hw_factor) / peak,
-            "traffic": None,
+            "traffic": measured_traffic("pair", args.workload, prec_name(main_p)),
+            "traffic_unit": "bytes per launch (HBM read + write)",
         },
     }
     if len(precs) > 1:

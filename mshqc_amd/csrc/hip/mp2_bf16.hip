@@ -21,7 +21,11 @@
 
 namespace {
 
-constexpr int QB = 128;   // Q block: exponent block and exact-FP32 span of the pair kernel
+// Q block of the (ia|jb) kernel: exponent block and exact-FP32 span.  Tier-0
+// class sums stay below 2^23 at 128; at 256 (sums up to 2^24) the energy and
+// gradient errors measured on gfx950 grew ~3x and ~12x, so the MFMA
+// accumulation is not exact in that last bit.
+constexpr int QB = 128;
 constexpr int KS = 16;    // Q per pair-kernel stage (one 32x32x16 MFMA K step)
 constexpr int RB = 32;    // LDS bytes per panel row, slice and pair-kernel stage
 constexpr int KC = 32;    // contraction chunk of the gradient GEMM (one 16x16x32 MFMA K step)
@@ -46,41 +50,50 @@ inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 // ------------------------------------------------------------- slicing
 
 // B[Q][o*v] (row-major, rows = q) -> Ps[S][Qp/16][o*v][16] and
-// Pe[Qp/128][o][vE] (exponent of row (i, a) over one 128-wide Q block; the
-// caller zeroes Pe, so the padding a >= v reads 0).  One workgroup per 64
-// rows x 128 q; the tile is transposed through LDS so both the FP64 reads and
-// the BF16 writes are contiguous.
+// Pe[Qp/QB][o][vE] (exponent of row (i, a) over one QB-wide Q block; the
+// caller zeroes Pe, so the padding a >= v reads 0).  One workgroup per 32
+// rows x QB q; the tile is transposed through LDS (dynamic, QB x 33 doubles)
+// so both the FP64 reads and the BF16 writes are contiguous.
+constexpr size_t slice_pair_lds() { return (size_t)QB * 33 * sizeof(double); }
+
 template <int S>
 __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restrict__ B, int Q, int ov, int v,
                                                          int vE, int o, int Qp, uint16_t* __restrict__ Ps,
                                                          int8_t* __restrict__ Pe) {
-    __shared__ double T[QB][65];
-    __shared__ int ex[64];
-    const int r0 = blockIdx.x * 64, q0 = blockIdx.y * QB, tid = threadIdx.x;
+    extern __shared__ __align__(16) char lds_raw[];
+    double (*T)[33] = reinterpret_cast<double (*)[33]>(lds_raw);
+    __shared__ int ex[32];
+    const int r0 = blockIdx.x * 32, q0 = blockIdx.y * QB, tid = threadIdx.x;
     {
-        const int c = tid & 63;
-        for (int k = 0; k < QB / 4; ++k) {
-            const int qq = (tid >> 6) + 4 * k;
+        const int c = tid & 31;
+        for (int k = 0; k < QB / 8; ++k) {
+            const int qq = (tid >> 5) + 8 * k;
             double x = 0.0;
             if (r0 + c < ov && q0 + qq < Q) x = B[(long)(q0 + qq) * ov + r0 + c];
             T[qq][c] = x;
         }
     }
     __syncthreads();
-    if (tid < 64) {
+    {
+        // 8 lanes per row, QB/8 values each, then a shuffle maximum.
+        const int row = tid >> 3, part = tid & 7;
         double mx = 0.0;
-        for (int qq = 0; qq < QB; ++qq) mx = fmax(mx, fabs(T[qq][tid]));
-        ex[tid] = block_exponent(mx);
-        const int r = r0 + tid;
-        if (r < ov) Pe[((long)blockIdx.y * o + r / v) * vE + r % v] = (int8_t)ex[tid];
+        for (int k = 0; k < QB / 8; ++k) mx = fmax(mx, fabs(T[part * (QB / 8) + k][row]));
+        for (int m = 1; m < 8; m <<= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
+        if (part == 0) {
+            const int e = block_exponent(mx);
+            ex[row] = e;
+            const int r = r0 + row;
+            if (r < ov) Pe[((long)blockIdx.y * o + r / v) * vE + r % v] = (int8_t)e;
+        }
     }
     __syncthreads();
     const long plane = (long)Qp * ov;
     // Item (q16 block kq, row, half): 8 consecutive q of one row; consecutive
-    // threads write consecutive 16-byte pieces of the 2 KiB (64 rows x 32 B)
+    // threads write consecutive 16-byte pieces of the 1 KiB (32 rows x 32 B)
     // block of one 16-wide Q step.
-    for (int idx = tid; idx < (QB / KS) * 64 * 2; idx += 256) {
-        const int kq = idx >> 7, row = (idx & 127) >> 1, half = idx & 1;
+    for (int idx = tid; idx < (QB / KS) * 32 * 2; idx += 256) {
+        const int kq = idx >> 6, row = (idx & 63) >> 1, half = idx & 1;
         if (r0 + row >= ov) continue;
         const int e = ex[row];
         uint16_t out[S][8];
@@ -453,6 +466,7 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 
     const int nst = Qp / KS;
     constexpr int SPB = QB / KS;  // stages per exponent block (even: buffer = stage parity)
+    static_assert(SPB % 2 == 0, "double buffer by stage parity");
 #pragma unroll
     for (int sl = 0; sl < S; ++sl) issue_slice(0, 0, sl);
     dma_barrier<0>();
@@ -460,7 +474,7 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
         // Exponents of this Q block, fetched early and without branches
         // (rows past the operand read a clamped row; those outputs are
         // discarded).  Row exponents come four rows to a dword: the tables
-        // are [Qb][o][vE] with vE a multiple of 64.
+        // are [Qp/QB][o][vE] with vE a multiple of 128.
         int erw[2][4], ecl[2];
 #pragma unroll
         for (int s = 0; s < 2; ++s)
@@ -498,7 +512,7 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
             }
             dma_barrier<0>();
         }
-        // FP64 flush of the block: (tier0 + tier1) + tier2, scaled by
+        // FP64 flush of the QB-wide block: (tier0 + tier1) + tier2, scaled by
         // 2^(e_row + e_col - 16).
 #pragma unroll
         for (int s = 0; s < 2; ++s)
@@ -740,8 +754,8 @@ struct SlicePair {
     static int run(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v, int vE, int Qp, uint16_t* Ps,
                    int8_t* Pe) {
         const int ov = o * v;
-        dim3 grid(ceil_div(ov, 64), Qp / QB);
-        slice_pair_kernel<S><<<grid, 256, 0, ctx->stream>>>(B, Q, ov, v, vE, o, Qp, Ps, Pe);
+        dim3 grid(ceil_div(ov, 32), Qp / QB);
+        slice_pair_kernel<S><<<grid, 256, slice_pair_lds(), ctx->stream>>>(B, Q, ov, v, vE, o, Qp, Ps, Pe);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -821,6 +835,8 @@ template <int S>
 struct AllowLds {
     static int run(mshqc_hip_ctx* ctx) {
         // Dynamic LDS above 64 KiB must be opted into per kernel.
+        hipFuncSetAttribute((const void*)slice_pair_kernel<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)slice_pair_lds());
         hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_RMP2, S>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, true));
         hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_SS, S>,
@@ -859,7 +875,7 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     o.qblocks = o.Qp / QB;
     const long ovl = (long)ol * vl, ovr = Br ? (long)orr * vr : 0;
     // Pair-kernel slices [S][Qp/16][o*v][16] and exponent tables
-    // [Qp/128][o][vE], vE = v rounded up to 128 (zero padding).
+    // [Qp/QB][o][vE], vE = v rounded up to 128 (zero padding).
     o.vEl = round_up(vl, 128);
     o.vEr = Br ? round_up(vr, 128) : o.vEl;
     const size_t bl = (size_t)S * ovl * o.Qp * 2, br = (size_t)S * ovr * o.Qp * 2;

@@ -1,13 +1,20 @@
 """Benchmark of the CD-MP2 energy + gradient on one MI355X.
 
-Workload (default): closed-shell CD-RMP2 energy plus gradient
+Workload (default, `mp2_o100`): closed-shell CD-RMP2 energy plus gradient
 (dE/dB^Q_ia and dE/deps_p) at Q=4000, o=100, v=1100 on seeded synthetic
 B^Q_ia and orbital energies, inputs resident in HBM before timing starts.
-One step = one full E+grad evaluation.  Both the FP64 form and the
-split-BF16 form (when requested/available) are timed; the JSON line reports
-the form named by --precision and carries the other as a comparison.
+One step = one full E+grad evaluation.  The split-BF16 form (5 slices) is
+timed and reported; the FP64 form is timed beside it as the comparison.
 
-Prints ONE JSON line on rank 0 (this build runs on one GPU only).
+Other workloads (`--workload`):
+  ump2_o120  CD-UMP2 energy + gradient, o_a=120, v_a=1380, o_b=118,
+             v_b=1382, Q=5600 (same-spin and opposite-spin blocks)
+  jk600      Fock J/K build from L^Q (n=600, Q=2400): general-P J + K, the
+             occupied-factor K (C_occ 600x60) and the AO->MO transform to
+             B^Q_ia; one step = those three calls
+  mp2_o60, mp2_o40  smaller CD-RMP2 E+grad workloads
+
+Prints ONE JSON line (this build runs on one GPU only).
 """
 from __future__ import annotations
 
@@ -25,6 +32,9 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
+METRIC = "CD-MP2 E+grad seconds & TFLOP/s, split-BF16 vs FP64 on MI355X at <=1 uEh error"
+DATA = ("synthetic (seeded generators: B=0.05 N(0,1) exp(-2a/v), eps_i~U[-1.5,-0.3], "
+        "eps_a=0.1+3(a/v)^1.5; L=sym(0.1 N(0,1) exp(-|mu-nu|/50)), C_occ=QR(N(0,1)))")
 
 
 def log(*a):
@@ -48,16 +58,23 @@ def rmp2_work(Q, o, v):
     return o * (o + 1) / 2 * v * v * 2.0 * Q
 
 
-def time_steps(fn, steps, warmup, torch):
-    for _ in range(warmup):
-        fn()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    out = None
-    for _ in range(steps):
-        out = fn()
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps, out
+def ump2_work(Q, oa, va, ob, vb):
+    """[oa(oa-1)/2 va^2 + ob(ob-1)/2 vb^2 + oa ob va vb] 2Q."""
+    return (oa * (oa - 1) / 2 * va * va + ob * (ob - 1) / 2 * vb * vb + oa * ob * va * vb) * 2.0 * Q
+
+
+def measured_traffic(kernel: str, workload: str, precision: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the calibration kernel), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}_{precision}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(B, eo, ev, npairs, seed=7):
@@ -84,18 +101,200 @@ def cpu_baseline(B, eo, ev, npairs, seed=7):
     }
 
 
-def measured_traffic(kernel: str, workload: str, precision: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE scaled by the calibration kernel), or None."""
-    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}_{precision}.md")
-    try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
-    except OSError:
-        return None
-    return None
+def timed(torch, ctx, fn, steps, warmup):
+    """Seconds per step of fn() and the per-kernel-class profile of the timed steps."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    prof = {k: ctx.profile_get(k) for k in (0, 1, 2)}
+    ctx.profile(False)
+    return sec, out, prof
+
+
+def calibrate(torch, mib):
+    """A plain streaming kernel of known size with a name of its own (sqrt):
+    it reads and writes mib MiB once each; PMC byte counters are scaled by it."""
+    nel = mib * (1 << 20) // 8
+    src = torch.full((nel,), 2.0, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.sqrt(src, out=dst)
+    torch.cuda.synchronize()
+    log(f"[bench] calibration sqrt: {nel * 8} bytes read, {nel * 8} bytes written")
+
+
+def base_line(args, sec, dtype, config, flop):
+    return {
+        "metric": METRIC,
+        "value": sec,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": DATA,
+        "config": config,
+        "tflops": flop / sec / 1e12 if flop else None,
+    }
+
+
+def mp2_line(args, results, precs, W3, config, energy_of, grads_of):
+    """JSON line of an MP2 E+grad workload (RMP2 or UMP2)."""
+    import torch
+
+    main_p = precs[0]
+    r = results[main_p]
+    steps = args.steps
+    npair_launch, pair_ms = r["prof"][0]
+    ngemm, gemm_ms = r["prof"][1]
+    pair_ms_step, gemm_ms_step = pair_ms / steps, gemm_ms / steps
+    W = W3 / 3
+    peak = FP64_PEAK_TFLOPS if main_p == 0 else BF16_PEAK_TFLOPS
+    pair_tflops = W / (pair_ms_step * 1e-3) / 1e12 if pair_ms_step > 0 else 0.0
+    hw_factor = 1 if main_p == 0 else main_p * (main_p + 1) // 2
+    line = base_line(args, r["sec"], prec_name(main_p), config, W3)
+    line["energy"] = energy_of(r["out"])
+    line["kernels"] = {
+        "pair_kernel": {"launches_per_step": npair_launch / steps, "ms_per_step": pair_ms_step,
+                        "algorithmic_flop_per_step": W},
+        "gemm": {"launches_per_step": ngemm / steps, "ms_per_step": gemm_ms_step,
+                 "algorithmic_flop_per_step": 2 * W},
+    }
+    line["roofline"] = {
+        "kernel": "pair-block (ia|jb) kernel with fused denominator epilogue",
+        "bound": "mfma",
+        "achieved": pair_tflops,
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": (pair_tflops * hw_factor) / peak,
+        "traffic": measured_traffic("pair", args.workload, prec_name(main_p)),
+        "traffic_unit": "bytes per launch (HBM read + write)",
+    }
+    if len(precs) > 1:
+        other = results[precs[1]]
+        Ea, Eb = energy_of(r["out"]), energy_of(other["out"])
+        ga, gb = grads_of(r["out"]), grads_of(other["out"])
+        num = sum(float(torch.linalg.norm(x - y).item()) ** 2 for x, y in zip(ga, gb)) ** 0.5
+        den = sum(float(torch.linalg.norm(y).item()) ** 2 for y in gb) ** 0.5
+        line["compare"] = {
+            "precision": prec_name(precs[1]),
+            "seconds": other["sec"],
+            "energy": Eb,
+            "abs_dE": abs(Ea - Eb),
+            "rel_dE": abs(Ea - Eb) / abs(Eb),
+            "speedup": other["sec"] / r["sec"],
+            "grad_rel_err": num / den,
+        }
+    return line
+
+
+def run_mp2(args, torch, hip, syn, w):
+    ctx = hip.context(0)
+    precs = [parse_precision(args.precision)]
+    if args.compare:
+        precs.append(parse_precision(args.compare))
+    if w["kind"] == "rmp2":
+        Q, o, v = w["Q"], w["o"], w["v"]
+        log(f"[bench] generating {args.workload}: Q={Q} o={o} v={v}")
+        B, eo, ev = syn.workload(args.workload)
+        dB, deo, dev = (torch.from_numpy(x).cuda() for x in (B, eo, ev))
+        W3 = 3 * rmp2_work(Q, o, v)
+        config = {"workload": f"{args.workload}: CD-RMP2 energy + gradient (dE/dB, dE/deps)",
+                  "Q": Q, "o": o, "v": v, "parallelism": "single GPU"}
+
+        def step_of(p):
+            return lambda: ctx.rmp2_grad(dB, deo, dev, precision=p)
+
+        def energy_of(out):
+            return out[0]
+
+        def grads_of(out):
+            return [out[1]]
+    else:
+        Q, oa, va, ob, vb = w["Q"], w["oa"], w["va"], w["ob"], w["vb"]
+        log(f"[bench] generating {args.workload}: Q={Q} oa={oa} va={va} ob={ob} vb={vb}")
+        (Ba, eoa, eva), (Bb, eob, evb) = syn.workload(args.workload)
+        dev_in = [torch.from_numpy(x).cuda() for x in (Ba, Bb, eoa, eva, eob, evb)]
+        del Ba, Bb
+        W3 = 3 * ump2_work(Q, oa, va, ob, vb)
+        config = {"workload": f"{args.workload}: CD-UMP2 energy + gradient (same and opposite spin)",
+                  "Q": Q, "o_alpha": oa, "v_alpha": va, "o_beta": ob, "v_beta": vb,
+                  "parallelism": "single GPU"}
+
+        def step_of(p):
+            return lambda: ctx.ump2_grad(*dev_in, precision=p)
+
+        def energy_of(out):
+            return float(sum(out[0]))
+
+        def grads_of(out):
+            return [out[1][0], out[1][1]]
+    if args.copy_calibration:
+        calibrate(torch, args.copy_calibration)
+    results = {}
+    for p in precs:
+        log(f"[bench] E+grad {prec_name(p)}: warmup {args.warmup}, steps {args.steps}")
+        sec, out, prof = timed(torch, ctx, step_of(p), args.steps, args.warmup)
+        log(f"[bench] {prec_name(p)}: {sec:.4f} s/step, E = {energy_of(out)!r}, pair kernel {prof[0]}, "
+            f"gemm {prof[1]}")
+        results[p] = dict(sec=sec, out=out, prof=prof)
+    config["precision"] = prec_name(precs[0])
+    line = mp2_line(args, results, precs, W3, config, energy_of, grads_of)
+    if w["kind"] == "rmp2" and not args.no_cpu:
+        log(f"[bench] cpu baseline on {args.cpu_pairs} sampled pairs")
+        line["cpu_baseline"] = cpu_baseline(B, eo, ev, args.cpu_pairs)
+    return line
+
+
+def run_jk(args, torch, hip, syn, w):
+    """General-P J + K, occupied-factor J + K and the AO->MO transform on L^Q."""
+    ctx = hip.context(0)
+    Q, n, o = w["Q"], w["n"], w["o"]
+    log(f"[bench] generating {args.workload}: Q={Q} n={n} o={o}")
+    L, C, P = syn.workload(args.workload)
+    dL = torch.from_numpy(L).cuda()
+    del L
+    dC, dP = torch.from_numpy(C).cuda(), torch.from_numpy(P).cuda()
+    Cvir = torch.from_numpy(np.ascontiguousarray(np.linalg.qr(
+        np.random.default_rng(w["seed"] + 1).standard_normal((n, n - o)))[0])).cuda()
+    parts = {
+        "jk_general_P": (lambda: ctx.jk(dL, dP, screen=True), 4.0 * Q * n ** 3),
+        "jk_occupied_factor": (lambda: ctx.jk_factor(dL, dC), 4.0 * Q * n * n * o),
+        "transform": (lambda: ctx.transform(dL, dC, Cvir), 2.0 * Q * n * n * o + 2.0 * Q * n * o * (n - o)),
+    }
+    if args.copy_calibration:
+        calibrate(torch, args.copy_calibration)
+    out = {}
+    total = 0.0
+    for name, (fn, flops) in parts.items():
+        sec, _, prof = timed(torch, ctx, fn, args.steps, args.warmup)
+        j_ms = prof[2][1] / args.steps
+        out[name] = {"seconds": sec, "flop": flops, "tflops": flops / sec / 1e12,
+                     "j_kernel_ms": j_ms if j_ms > 0 else None}
+        total += sec
+        log(f"[bench] {name}: {sec * 1e3:.2f} ms/step, {flops / sec / 1e12:.1f} TFLOP/s, J kernels {prof[2]}")
+    jb = 16.0 * Q * n * n  # two passes over L
+    j_ms = out["jk_occupied_factor"]["j_kernel_ms"]
+    config = {"workload": f"{args.workload}: CD-RHF Fock pieces from L", "Q": Q, "n": n, "o": o,
+              "parallelism": "single GPU"}
+    line = base_line(args, total, "f64", config, sum(f for _, f in parts.values()))
+    line["kernels"] = out
+    if j_ms:
+        achieved = jb / (j_ms * 1e-3) / 1e9
+        line["roofline"] = {"kernel": "J: per-Q density dot products + accumulation (two passes over L)",
+                            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    return line
 
 
 def main():
@@ -109,7 +308,7 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
-                    help="MiB moved by one plain device copy before timing (a known-size "
+                    help="MiB moved by one plain device kernel before timing (a known-size "
                          "kernel for scaling PMC byte counters)")
     args = ap.parse_args()
 
@@ -124,116 +323,10 @@ def main():
     from mshqc_amd import hip, synthetic as syn
 
     w = syn.WORKLOADS[args.workload]
-    assert w["kind"] == "rmp2"
-    Q, o, v = w["Q"], w["o"], w["v"]
-    log(f"[bench] generating {args.workload}: Q={Q} o={o} v={v}")
-    B, eo, ev = syn.workload(args.workload)
-    dB = torch.from_numpy(B).cuda()
-    deo = torch.from_numpy(eo).cuda()
-    dev = torch.from_numpy(ev).cuda()
-    ctx = hip.context(0)
-    W = rmp2_work(Q, o, v)
-    if args.copy_calibration:
-        # A plain streaming kernel of known size with a name of its own
-        # (sqrt): it reads and writes nel * 8 bytes once each.
-        nel = args.copy_calibration * (1 << 20) // 8
-        src = torch.full((nel,), 2.0, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        torch.sqrt(src, out=dst)
-        torch.cuda.synchronize()
-        log(f"[bench] calibration sqrt: {nel * 8} bytes read, {nel * 8} bytes written")
-        del src, dst
-
-    results = {}
-    precs = [parse_precision(args.precision)]
-    if args.compare:
-        precs.append(parse_precision(args.compare))
-    for p in precs:
-        log(f"[bench] E+grad {prec_name(p)}: warmup {args.warmup}, steps {args.steps}")
-
-        def step():
-            return ctx.rmp2_grad(dB, deo, dev, precision=p)
-
-        for _ in range(args.warmup):
-            step()
-        torch.cuda.synchronize()
-        ctx.profile(True)
-        t0 = time.perf_counter()
-        out = None
-        for _ in range(args.steps):
-            out = step()
-        torch.cuda.synchronize()
-        sec = (time.perf_counter() - t0) / args.steps
-        prof = {k: ctx.profile_get(k) for k in (0, 1, 2)}
-        ctx.profile(False)
-        E = out[0]
-        log(f"[bench] {prec_name(p)}: {sec:.4f} s/step, E = {E!r}, pair kernel {prof[0]}, gemm {prof[1]}")
-        results[p] = dict(sec=sec, E=E, prof=prof, dEdB=out[1], deo=out[2], dev=out[3])
-
-    main_p = precs[0]
-    r = results[main_p]
-    sec = r["sec"]
-    npair_launch, pair_ms = r["prof"][0]
-    ngemm, gemm_ms = r["prof"][1]
-    steps = args.steps
-    pair_ms_step = pair_ms / steps
-    gemm_ms_step = gemm_ms / steps
-    # Dominant kernel: the pair-block (ia|jb) kernel. Algorithmic flops per step: W.
-    peak = FP64_PEAK_TFLOPS if main_p == 0 else BF16_PEAK_TFLOPS
-    pair_tflops = W / (pair_ms_step * 1e-3) / 1e12 if pair_ms_step > 0 else 0.0
-    hw_factor = 1 if main_p == 0 else main_p * (main_p + 1) // 2
-    line = {
-        "metric": "CD-MP2 E+grad seconds & TFLOP/s, split-BF16 vs FP64 on MI355X at <=1 uEh error",
-        "value": sec,
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": sec * 1e3,
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": prec_name(main_p),
-        "data": "synthetic (seeded generators: B=0.05 N(0,1) exp(-2a/v), eps_i~U[-1.5,-0.3], "
-                "eps_a=0.1+3(a/v)^1.5)",
-        "config": {"workload": f"{args.workload}: CD-RMP2 energy + gradient (dE/dB, dE/deps)",
-                   "Q": Q, "o": o, "v": v, "precision": prec_name(main_p),
-                   "parallelism": "single GPU"},
-        "tflops": 3 * W / sec / 1e12,
-        "energy": r["E"],
-        "kernels": {
-            "pair_kernel": {"launches_per_step": npair_launch / steps, "ms_per_step": pair_ms_step,
-                            "algorithmic_flop_per_step": W},
-            "gemm": {"launches_per_step": ngemm / steps, "ms_per_step": gemm_ms_step,
-                     "algorithmic_flop_per_step": 2 * W},
-        },
-        "roofline": {
-            "kernel": "pair-block (ia|jb) kernel with fused denominator epilogue",
-            "bound": "mfma",
-            "achieved": pair_tflops,
-            "peak": peak,
-            "unit": "TFLOP/s",
-            "frac": (pair_tflops * hw_factor) / peak,
-            "traffic": measured_traffic("pair", args.workload, prec_name(main_p)),
-            "traffic_unit": "bytes per launch (HBM read + write)",
-        },
-    }
-    if len(precs) > 1:
-        other = results[precs[1]]
-        dE = r["E"] - other["E"]
-        line["compare"] = {
-            "precision": prec_name(precs[1]),
-            "seconds": other["sec"],
-            "energy": other["E"],
-            "abs_dE": abs(dE),
-            "rel_dE": abs(dE) / abs(other["E"]),
-            "speedup": other["sec"] / sec,
-            "grad_rel_err": float(torch.linalg.norm(r["dEdB"] - other["dEdB"]).item()
-                                  / torch.linalg.norm(other["dEdB"]).item()),
-        }
-    if not args.no_cpu:
-        log(f"[bench] cpu baseline on {args.cpu_pairs} sampled pairs")
-        line["cpu_baseline"] = cpu_baseline(B, eo, ev, args.cpu_pairs)
+    if w["kind"] in ("rmp2", "ump2"):
+        line = run_mp2(args, torch, hip, syn, w)
+    else:
+        line = run_jk(args, torch, hip, syn, w)
     print(json.dumps(line), flush=True)
 
 

@@ -607,13 +607,15 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const int m0 = mt * GT, n0 = ntile * GT;
     // DMA load u of this wave covers operand P, slice sl and panel rows
-    // 16*rb .. 16*rb+15 (id = wave + 4u enumerates (P, sl, rb)); lanes of rows
-    // past the matrix load nothing, and those LDS rows only feed discarded
-    // outputs.
+    // 16*rb .. 16*rb+15 (id = wave + 4u enumerates (P, sl, rb)).  Rows past
+    // the matrix are clamped to its last row (those LDS rows only feed
+    // discarded outputs): every wave then issues exactly PER loads per
+    // stage, which the vmcnt(PER) stage barrier relies on.  (A load skipped
+    // for an all-out-of-range block would let the barrier pass before the
+    // previous stage had landed.)
     const int wv = __builtin_amdgcn_readfirstlane(wave);
     const int lrow = lane >> 2;
-    const long laneA = (long)lrow * A.ld + 8 * lane_seg(lane);
-    const long laneX = (long)lrow * X.ld + 8 * lane_seg(lane);
+    const int lseg8 = 8 * lane_seg(lane);
     // Loads u in [4g, 4g+4) of a stage; a stage's loads are spread over its
     // S MFMA groups (PER = 4S).
     auto issue_part = [&](int k0, int buf, int g) {
@@ -622,10 +624,9 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
             const int id = wv + 4 * u;
             const int P = id / (8 * S), sl = (id / 8) % S, rb = id % 8;
             const SlicedMat& M = P ? X : A;
-            const int row = (P ? n0 : m0) + 16 * rb;
-            if (row + lrow < M.rows)
-                dma16(M.s + sl * M.plane + (long)row * M.ld + k0 + (P ? laneX : laneA),
-                      lds + buf * STAGE + (P * S + sl) * PANEL + 1024 * rb);
+            const int r = min((P ? n0 : m0) + 16 * rb + lrow, M.rows - 1);
+            dma16(M.s + sl * M.plane + (long)r * M.ld + k0 + lseg8,
+                  lds + buf * STAGE + (P * S + sl) * PANEL + 1024 * rb);
         }
     };
     v4f acc[2][4][4];

@@ -1,0 +1,52 @@
+"""Bit-reproducibility: two identical calls return bitwise-equal energies and
+gradients (every reduction on the path has a fixed order, no atomics), in
+the FP64 and the split-BF16 forms, with the gradient cut into several
+occupied block pairs including diagonal ones.
+
+The larger RMP2 case has Q = 2000, so the last 128-row tile of the gradient
+GEMMs holds whole 16-row load blocks past the matrix, and its operands come
+from HBM.  The DMA stage pipeline of that GEMM once skipped such loads,
+which let its stage barrier pass early and made the gradient differ from
+call to call at this size."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import to_dev
+from mshqc_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision,shape", [(0, (600, 9, 260)), (5, (600, 9, 260)),
+                                             (5, (2000, 30, 700))])
+def test_rmp2_grad_bitwise_repeatable(ctx, precision, shape):
+    Q, o, v = shape
+    B, eo, ev = syn.rmp2_inputs(Q, o, v, 90)
+    dB, deo, dev = to_dev(B, eo, ev)
+    ctx.set_zblock((o // 3) * v * (o // 3) * v)  # three occupied blocks
+    try:
+        runs = [ctx.rmp2_grad(dB, deo, dev, precision=precision) for _ in range(2)]
+    finally:
+        ctx.set_zblock(1 << 27)
+    (E1, g1, o1, v1), (E2, g2, o2, v2) = runs
+    print(f"precision {precision}: E {E1!r} {E2!r}", flush=True)
+    assert E1 == E2
+    assert torch.equal(g1, g2)
+    assert torch.equal(o1, o2)
+    assert torch.equal(v1, v2)
+
+
+@pytest.mark.parametrize("precision", [0, 5])
+def test_ump2_grad_bitwise_repeatable(ctx, precision):
+    (Ba, eoa, eva), (Bb, eob, evb) = syn.ump2_inputs(500, 5, 200, 4, 210, 91)
+    args = to_dev(Ba, Bb, eoa, eva, eob, evb)
+    ctx.set_zblock(2 * 210 * 2 * 210)
+    try:
+        runs = [ctx.ump2_grad(*args, precision=precision) for _ in range(2)]
+    finally:
+        ctx.set_zblock(1 << 27)
+    (E1, g1), (E2, g2) = runs
+    assert E1 == E2
+    for a, b in zip(g1, g2):
+        assert torch.equal(a, b)

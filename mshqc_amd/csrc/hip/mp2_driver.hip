@@ -54,38 +54,53 @@ __global__ void occ_reduce_kernel(const int2* __restrict__ pairs, const double* 
 // dE/deps_vir: row sums feed the left virtual index, column sums the right.
 // Only the tiles that touch the index's 64-block are visited: in the
 // symmetric modes the tile list holds (A,B), A <= B, in row-major order; in
-// the opposite-spin mode it holds all (A,B) with B fastest.
-__global__ void vir_reduce_kernel(int ntA, int ntB, int npairs, const double* __restrict__ vrow,
-                                  const double* __restrict__ vcol, double* dev_l, int vl,
-                                  double* dev_r, int vr, int sym) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+// the opposite-spin mode it holds all (A,B) with B fastest.  A workgroup
+// covers 16 virtual indices; its 16 thread rows split the pairs into fixed
+// chunks whose sums are combined in chunk order (deterministic).
+constexpr int VR_IDX = 16, VR_CHUNKS = 16;
+__global__ __launch_bounds__(VR_IDX * VR_CHUNKS) void vir_reduce_kernel(
+    int ntA, int ntB, int npairs, const double* __restrict__ vrow, const double* __restrict__ vcol,
+    double* dev_l, int vl, double* dev_r, int vr, int sym) {
+    __shared__ double sl[VR_CHUNKS][VR_IDX], sr[VR_CHUNKS][VR_IDX];
+    const int li = threadIdx.x % VR_IDX, c = threadIdx.x / VR_IDX;
+    const int a = blockIdx.x * VR_IDX + li;
     const int blk = a / TILE, off = a % TILE;
     const int ntiles = sym ? ntA * (ntA + 1) / 2 : ntA * ntB;
     auto sym_index = [ntA](int A, int B) { return A * ntA - A * (A - 1) / 2 + (B - A); };
+    const int per = (npairs + VR_CHUNKS - 1) / VR_CHUNKS;
+    const int q0 = c * per, q1 = min(npairs, q0 + per);
+    double accl = 0.0, accr = 0.0;
     if (sym) {
-        if (a >= vl) return;
-        double acc = 0.0;
-        for (int q = 0; q < npairs; ++q) {
-            const long base = (long)q * ntiles;
-            for (int B = blk; B < ntA; ++B) acc += vrow[(base + sym_index(blk, B)) * TILE + off];
-            for (int A = 0; A <= blk; ++A) acc += vcol[(base + sym_index(A, blk)) * TILE + off];
-        }
-        dev_l[a] += acc;
+        if (a < vl)
+            for (int q = q0; q < q1; ++q) {
+                const long base = (long)q * ntiles;
+                for (int B = blk; B < ntA; ++B) accl += vrow[(base + sym_index(blk, B)) * TILE + off];
+                for (int A = 0; A <= blk; ++A) accl += vcol[(base + sym_index(A, blk)) * TILE + off];
+            }
     } else {
-        if (a < vl && dev_l) {
-            double acc = 0.0;
-            for (int q = 0; q < npairs; ++q)
+        if (a < vl && dev_l)
+            for (int q = q0; q < q1; ++q)
                 for (int B = 0; B < ntB; ++B)
-                    acc += vrow[((long)q * ntiles + (long)blk * ntB + B) * TILE + off];
-            dev_l[a] += acc;
-        }
-        if (a < vr && dev_r) {
-            double acc = 0.0;
-            for (int q = 0; q < npairs; ++q)
+                    accl += vrow[((long)q * ntiles + (long)blk * ntB + B) * TILE + off];
+        if (a < vr && dev_r)
+            for (int q = q0; q < q1; ++q)
                 for (int A = 0; A < ntA; ++A)
-                    acc += vcol[((long)q * ntiles + (long)A * ntB + blk) * TILE + off];
-            dev_r[a] += acc;
-        }
+                    accr += vcol[((long)q * ntiles + (long)A * ntB + blk) * TILE + off];
+    }
+    sl[c][li] = accl;
+    sr[c][li] = accr;
+    __syncthreads();
+    if (c != 0) return;
+    double tl = 0.0, tr = 0.0;
+    for (int k = 0; k < VR_CHUNKS; ++k) {
+        tl += sl[k][li];
+        tr += sr[k][li];
+    }
+    if (sym) {
+        if (a < vl) dev_l[a] += tl;
+    } else {
+        if (a < vl && dev_l) dev_l[a] += tl;
+        if (a < vr && dev_r) dev_r[a] += tr;
     }
 }
 
@@ -293,7 +308,7 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
         }
         if (pr.devl || pr.devr) {
             const int nv = std::max(pr.vl, pr.vr);
-            vir_reduce_kernel<<<ceil_div(nv, 128), 128, 0, ctx->stream>>>(
+            vir_reduce_kernel<<<ceil_div(nv, VR_IDX), VR_IDX * VR_CHUNKS, 0, ctx->stream>>>(
                 ntA, ntB, sg.n, a.vrow, a.vcol, pr.devl, pr.devl ? pr.vl : 0, pr.devr,
                 pr.devr ? pr.vr : 0, sym ? 1 : 0);
             LAUNCH_CHECK(ctx);

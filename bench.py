@@ -257,7 +257,8 @@ def run_mp2(args, torch, hip, syn, w):
 
 
 def run_jk(args, torch, hip, syn, w):
-    """General-P J + K, occupied-factor J + K and the AO->MO transform on L^Q."""
+    """General-P J + K, occupied-factor J + K, restricted open-shell J + Ka + Kb
+    and the AO->MO transform on L^Q."""
     ctx = hip.context(0)
     Q, n, o = w["Q"], w["n"], w["o"]
     log(f"[bench] generating {args.workload}: Q={Q} n={n} o={o}")
@@ -270,6 +271,8 @@ def run_jk(args, torch, hip, syn, w):
     parts = {
         "jk_general_P": (lambda: ctx.jk(dL, dP, screen=True), 4.0 * Q * n ** 3),
         "jk_occupied_factor": (lambda: ctx.jk_factor(dL, dC), 4.0 * Q * n * n * o),
+        # restricted open shell, o alpha and o - 2 beta electrons (a triplet)
+        "jk_rohf": (lambda: ctx.jk_rohf(dL, dC, o, o - 2), 2.0 * Q * n * n * o + 2.0 * Q * n * n * (2 * o - 2)),
         "transform": (lambda: ctx.transform(dL, dC, Cvir), 2.0 * Q * n * n * o + 2.0 * Q * n * o * (n - o)),
     }
     if args.copy_calibration:
@@ -285,7 +288,7 @@ def run_jk(args, torch, hip, syn, w):
         log(f"[bench] {name}: {sec * 1e3:.2f} ms/step, {flops / sec / 1e12:.1f} TFLOP/s, J kernels {prof[2]}")
     jb = 16.0 * Q * n * n  # two passes over L
     j_ms = out["jk_occupied_factor"]["j_kernel_ms"]
-    config = {"workload": f"{args.workload}: CD-RHF Fock pieces from L", "Q": Q, "n": n, "o": o,
+    config = {"workload": f"{args.workload}: CD-RHF/ROHF Fock pieces from L", "Q": Q, "n": n, "o": o,
               "parallelism": "single GPU"}
     line = base_line(args, total, "f64", config, sum(f for _, f in parts.values()))
     line["kernels"] = out

@@ -24,9 +24,10 @@
  *   - precision: MSHQC_PREC_FP64 (0) computes in FP64 on FP64 MFMA.  A value
  *     k >= 2 selects the split-BF16 form with k BF16 slices per operand
  *     (k = 2 is the hi+lo three-product form); slice products with
- *     p + q <= k - 1 are kept, accumulated exactly in FP32 within 64-wide Q
- *     blocks and summed across blocks in FP64.  Denominators and every
- *     energy/gradient sum stay FP64.
+ *     p + q <= k - 1 are kept, the two leading product classes accumulated
+ *     exactly in FP32 within 128-wide Q (or contraction) blocks, and the
+ *     blocks summed in FP64.  Denominators and every energy/gradient sum
+ *     stay FP64.
  */
 #ifndef MSHQC_HIP_H
 #define MSHQC_HIP_H
@@ -104,6 +105,21 @@ int mshqc_hip_jk(mshqc_hip_ctx* ctx, const double* L, int Q, int n,
  */
 int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n,
                         const double* Y, int m, double* J, double* K);
+
+/*
+ * Restricted open-shell Coulomb and exchange: one spatial orbital set C
+ * (row-major, n x ldc), alpha electrons in columns [0, na), beta in [0, nb):
+ *   J  = sum_Q tr(L^Q (Pa + Pb)) L^Q,     Pa = C_a C_a^T, Pb = C_b C_b^T
+ *   Ka = sum_Q (L^Q C_a)(L^Q C_a)^T,      Kb = sum_Q (L^Q C_b)(L^Q C_b)^T
+ * Replaces CholeskyROHF::build_fock_cholesky (src/scf/cholesky_rohf.cc:
+ * 89-147), whose C_beta equals C_alpha on every iteration (:77-83, :235-236);
+ * the caller assembles F_a = H + J - Ka, F_b = H + J - Kb.  One half
+ * transform L^Q C_{0..max(na,nb)} serves both spins.  na = 0 or nb = 0 give
+ * zero matrices for that spin.
+ */
+int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n,
+                      const double* C, int ldc, int na, int nb, double* J,
+                      double* Ka, double* Kb);
 
 /* ---------------------------------------------------------- transforms */
 

@@ -97,6 +97,17 @@ int j_build(mshqc_hip_ctx* ctx, const double* L, int Q, long nn, const double* P
     return MSHQC_HIP_OK;
 }
 
+__global__ void fill_zero_kernel(double* __restrict__ x, long count) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) x[k] = 0.0;
+}
+
+int fill_zero(mshqc_hip_ctx* ctx, double* x, long count) {
+    fill_zero_kernel<<<(unsigned)((count + 255) / 256), 256, 0, ctx->stream>>>(x, count);
+    LAUNCH_CHECK(ctx);
+    return MSHQC_HIP_OK;
+}
+
 // Largest Q chunk whose n x m temporaries fit in `cap` doubles.
 int q_chunk(int Q, long per_q, size_t cap) {
     long c = (long)(cap / (size_t)std::max(per_q, 1L));
@@ -254,6 +265,101 @@ int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const
         g2.accumulate = q0 > 0;
         { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
             if (rc) return rc;
+    }
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if ((rc = ctx_finish(ctx))) return rc;
+    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+    return MSHQC_HIP_OK;
+}
+
+int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* C, int ldc,
+                      int na, int nb, double* J, double* Ka, double* Kb) {
+    // cholesky_rohf.cc:89-147.  C_beta = C_alpha there, so the beta occupied
+    // block is the first nb columns of the alpha one and X^Q = L^Q C_{0..m}
+    // (m = max(na, nb)) feeds both exchange matrices.
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, L && C && J && Ka && Kb, "jk_rohf: null pointer");
+    CHECK_ARG(ctx, Q > 0 && n > 0 && na >= 0 && nb >= 0, "jk_rohf: bad dimensions");
+    const int m = std::max(na, nb);
+    CHECK_ARG(ctx, ldc >= std::max(m, 1), "jk_rohf: ldc smaller than max(na, nb)");
+    hipEventRecord(ctx->ev0, ctx->stream);
+    const long nn = (long)n * n, nm = (long)n * std::max(m, 1);
+    if (m == 0) {
+        if ((rc = fill_zero(ctx, J, nn)) || (rc = fill_zero(ctx, Ka, nn)) || (rc = fill_zero(ctx, Kb, nn)))
+            return rc;
+    } else {
+        const int qc = q_chunk(Q, nm, kTmpCap);
+        double* tmp = static_cast<double*>(
+            workspace(ctx, WS_TMP, sizeof(double) * ((size_t)qc * nm + nn + (size_t)Q + 64)));
+        if (!tmp) return MSHQC_HIP_ERR_ALLOC;
+        double* X = tmp;
+        double* P = tmp + (size_t)qc * nm;
+        double* gam = P + nn;
+        // P_tot = C_a C_a^T + C_b C_b^T
+        bool have_p = false;
+        for (int cols : {na, nb}) {
+            if (cols == 0) continue;
+            GemmF64 gp;
+            gp.A = C;
+            gp.lda = ldc;
+            gp.X = C;
+            gp.ldx = ldc;
+            gp.transX = true;
+            gp.C = P;
+            gp.ldc = n;
+            gp.M = gp.N = n;
+            gp.K = cols;
+            gp.accumulate = have_p;
+            if ((rc = gemm_f64(ctx, gp))) return rc;
+            have_p = true;
+        }
+        if ((rc = j_build(ctx, L, Q, nn, P, J, gam))) return rc;
+        if (na == 0 && (rc = fill_zero(ctx, Ka, nn))) return rc;
+        if (nb == 0 && (rc = fill_zero(ctx, Kb, nn))) return rc;
+        for (int q0 = 0; q0 < Q; q0 += qc) {
+            const int nq = std::min(qc, Q - q0);
+            // X_q = L^q C_{0..m}
+            GemmF64 g1;
+            g1.A = L + (long)q0 * nn;
+            g1.lda = n;
+            g1.sA = nn;
+            g1.X = C;
+            g1.ldx = ldc;
+            g1.sX = 0;
+            g1.C = X;
+            g1.ldc = m;
+            g1.sC = nm;
+            g1.M = n;
+            g1.N = m;
+            g1.K = n;
+            g1.batch = nq;
+            { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g1); }
+            if (rc) return rc;
+            // K_s (+)= sum_q X_q[:, 0..m_s) X_q[:, 0..m_s)^T
+            for (int s = 0; s < 2; ++s) {
+                const int ms = s == 0 ? na : nb;
+                if (ms == 0) continue;
+                GemmF64 g2;
+                g2.A = X;
+                g2.lda = m;
+                g2.sA = nm;
+                g2.X = X;
+                g2.ldx = m;
+                g2.sX = nm;
+                g2.transX = true;
+                g2.C = s == 0 ? Ka : Kb;
+                g2.ldc = n;
+                g2.M = g2.N = n;
+                g2.K = ms;
+                g2.batch = nq;
+                g2.reduce_batch = true;
+                g2.splits = gemm_splits(n, n, nq);
+                g2.accumulate = q0 > 0;
+                { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
+                if (rc) return rc;
+            }
+        }
     }
     hipEventRecord(ctx->ev1, ctx->stream);
     if ((rc = ctx_finish(ctx))) return rc;

@@ -276,6 +276,24 @@ PYBIND11_MODULE(_mshqc, m) {
             py::arg("vectors"))
         .def("compute_s_squared", &CholeskyUHF::compute_s_squared);
 
+    // ---------------------------------------------------------- ROHF
+    // bindings.cc:249-275 of the reference
+    py::class_<CholeskyROHFConfig, SCFConfig>(m, "CholeskyROHFConfig")
+        .def(py::init<>())
+        .def_readwrite("cholesky_threshold", &CholeskyROHFConfig::cholesky_threshold)
+        .def_readwrite("screen_exchange", &CholeskyROHFConfig::screen_exchange);
+
+    py::class_<CholeskyROHF>(m, "CholeskyROHF")
+        .def(py::init<const Molecule&, const BasisSet&, std::shared_ptr<IntegralEngine>, int, int,
+                      const CholeskyROHFConfig&>(),
+             py::arg("molecule"), py::arg("basis"), py::arg("integrals"), py::arg("n_alpha"), py::arg("n_beta"),
+             py::arg("config") = CholeskyROHFConfig())
+        .def(py::init<const Molecule&, const BasisSet&, std::shared_ptr<IntegralEngine>, int, int,
+                      const CholeskyROHFConfig&, const integrals::CholeskyERI&>(),
+             py::arg("molecule"), py::arg("basis"), py::arg("integrals"), py::arg("n_alpha"), py::arg("n_beta"),
+             py::arg("config"), py::arg("existing_cholesky"))
+        .def("compute", &CholeskyROHF::compute);
+
     // ----------------------------------------------------------- MP2
     py::class_<CholeskyRMP2Config>(m, "CholeskyRMP2Config")
         .def(py::init<>())
@@ -343,5 +361,13 @@ PYBIND11_MODULE(_mshqc, m) {
         Mat V;
         eigh(np_to_mat(A), w, V);
         return py::make_tuple(vec_to_np(w), mat_to_np(V));
+    });
+    m.def("linalg_solve", [](const arr& A, const arr& b) {
+        const Mat M = np_to_mat(A);
+        auto bb = b.unchecked<1>();
+        if (M.rows != M.cols || bb.shape(0) != M.rows) throw std::invalid_argument("linalg_solve: shape mismatch");
+        Vec v(M.rows);
+        for (int i = 0; i < M.rows; ++i) v[i] = bb(i);
+        return vec_to_np(solve(M, v));
     });
 }

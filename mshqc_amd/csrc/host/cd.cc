@@ -3,10 +3,12 @@
 // function; the Fock pieces and the MP2 pair work are C-ABI calls.
 #include "cd.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <iostream>
 #include <mutex>
+#include <string>
 
 namespace mshqc {
 
@@ -457,6 +459,148 @@ double CholeskyUHF::compute_s_squared() const {
     const double sz = 0.5 * (n_alpha_ - n_beta_);
     Mat Sab = matmul(matmul_tn(left_cols(Ca_, n_alpha_), S_), left_cols(Cb_, n_beta_));
     return sz * (sz + 1.0) + n_beta_ - dot(Sab, Sab);
+}
+
+// ================================================================ CD-ROHF
+
+CholeskyROHF::CholeskyROHF(const Molecule& mol, const BasisSet& basis,
+                           std::shared_ptr<IntegralEngine> integrals, int n_alpha, int n_beta,
+                           const CholeskyROHFConfig& config)
+    : mol_(mol), basis_(basis), integrals_(std::move(integrals)), config_(config),
+      nbasis_(basis.n_basis_functions()), n_alpha_(n_alpha), n_beta_(n_beta),
+      chol_(config.cholesky_threshold) {
+    // cholesky_rohf.cc:19-29: the decomposition waits for compute()
+    if (n_alpha < 0 || n_beta < 0 || n_alpha > nbasis_ || n_beta > nbasis_)
+        throw std::runtime_error("CholeskyROHF: occupation numbers out of range");
+}
+
+CholeskyROHF::CholeskyROHF(const Molecule& mol, const BasisSet& basis,
+                           std::shared_ptr<IntegralEngine> integrals, int n_alpha, int n_beta,
+                           const CholeskyROHFConfig& config, const integrals::CholeskyERI& existing_cholesky)
+    : mol_(mol), basis_(basis), integrals_(std::move(integrals)), config_(config),
+      nbasis_(basis.n_basis_functions()), n_alpha_(n_alpha), n_beta_(n_beta), chol_(existing_cholesky) {
+    // cholesky_rohf.cc:34-51: a copy of the caller's vectors (the device
+    // copy, once uploaded, is shared with it)
+    if (n_alpha < 0 || n_beta < 0 || n_alpha > nbasis_ || n_beta > nbasis_)
+        throw std::runtime_error("CholeskyROHF: occupation numbers out of range");
+    if (config_.print_level > 0)
+        std::printf("\n[CholeskyROHF] Initializing with REUSED vectors.\n  Vectors provided: %d\n",
+                    chol_.n_vectors());
+}
+
+void CholeskyROHF::init_integrals() {
+    // cholesky_rohf.cc:56-75
+    S_ = integrals_->compute_overlap();
+    H_ = add(integrals_->compute_kinetic(), integrals_->compute_nuclear());
+    if (chol_.n_vectors() == 0) {
+        if (config_.print_level > 0) std::printf("[CholeskyROHF] Decomposing Integrals...\n");
+        chol_.decompose(integrals_->compute_eri());
+    }
+    if (chol_.n_basis() != nbasis_)
+        throw std::runtime_error("CholeskyROHF: Cholesky vectors do not match the basis");
+    X_ = inverse_sqrt(S_);
+}
+
+void CholeskyROHF::solve_fock(const Mat& F) {
+    Mat Fp = matmul(matmul_tn(X_, F), X_);
+    Mat U;
+    eigh(Fp, eps_, U);
+    C_ = matmul(X_, U);
+}
+
+Mat CholeskyROHF::build_density(int n_occ) const {
+    if (n_occ == 0) return Mat(nbasis_, nbasis_);
+    Mat Co = left_cols(C_, n_occ);
+    return matmul_nt(Co, Co);
+}
+
+void CholeskyROHF::initial_guess() {
+    // cholesky_rohf.cc:77-83: core-Hamiltonian guess; the reference solves it
+    // once per spin, which gives the same orbitals twice
+    solve_fock(H_);
+    Pa_ = build_density(n_alpha_);
+    Pb_ = build_density(n_beta_);
+}
+
+void CholeskyROHF::build_fock_cholesky() {
+    // cholesky_rohf.cc:89-147: F_s = H + J[Pa + Pb] - K_s with the exchange
+    // from the occupied columns of the shared orbitals
+    const int n = nbasis_, Q = chol_.n_vectors();
+    Fa_ = H_;
+    Fb_ = H_;
+    if (n_alpha_ == 0 && n_beta_ == 0) return;
+    const int m = std::max(n_alpha_, n_beta_);
+    auto dC = to_device(left_cols(C_, m));
+    device::Buffer dJ((size_t)n * n), dKa((size_t)n * n), dKb((size_t)n * n);
+    device::check(mshqc_hip_jk_rohf(device::ctx(), chol_.device_vectors(), Q, n, dC->get(), m, n_alpha_,
+                                    n_beta_, dJ.get(), dKa.get(), dKb.get()),
+                  "mshqc_hip_jk_rohf");
+    Mat J(n, n), Ka(n, n), Kb(n, n);
+    dJ.download(J.data(), J.size());
+    dKa.download(Ka.data(), Ka.size());
+    dKb.download(Kb.data(), Kb.size());
+    Fa_ = add(add(H_, J), Ka, -1.0);
+    Fb_ = add(add(H_, J), Kb, -1.0);
+}
+
+double CholeskyROHF::compute_energy() const {
+    // cholesky_rohf.cc:159-165 (electronic energy)
+    return 0.5 * dot(Pa_, add(H_, Fa_)) + 0.5 * dot(Pb_, add(H_, Fb_));
+}
+
+SCFResult CholeskyROHF::compute() {
+    // cholesky_rohf.cc:184-276
+    if (config_.print_level > 0)
+        std::printf("\n%s\n  Cholesky-ROHF SCF Calculation\n%s\n", std::string(60, '=').c_str(),
+                    std::string(60, '=').c_str());
+    init_integrals();
+    initial_guess();
+    DIIS diis_a(config_.diis_max_vectors), diis_b(config_.diis_max_vectors);
+    const double e_nuc = mol_.nuclear_repulsion_energy();
+    const double na = n_alpha_, nb = n_beta_;
+    int iter = 0;
+    bool converged = false;
+    while (iter < config_.max_iterations) {
+        ++iter;
+        energy_old_ = energy_;
+        build_fock_cholesky();
+        energy_ = compute_energy();
+        Mat err_a = add(matmul(matmul(Fa_, Pa_), S_), matmul(matmul(S_, Pa_), Fa_), -1.0);
+        Mat err_b = add(matmul(matmul(Fb_, Pb_), S_), matmul(matmul(S_, Pb_), Fb_), -1.0);
+        if (std::sqrt(dot(err_a, err_a)) < config_.diis_threshold) {
+            diis_a.add_iteration(Fa_, err_a);
+            if (diis_a.can_extrapolate()) Fa_ = diis_a.extrapolate();
+        }
+        if (std::sqrt(dot(err_b, err_b)) < config_.diis_threshold) {
+            diis_b.add_iteration(Fb_, err_b);
+            if (diis_b.can_extrapolate()) Fb_ = diis_b.extrapolate();
+        }
+        // occupation-weighted average (cholesky_rohf.cc:228-232)
+        Mat F_eff = scale(add(scale(Fa_, na), Fb_, nb), 1.0 / (na + nb));
+        solve_fock(F_eff);
+        Pa_ = build_density(n_alpha_);
+        Pb_ = build_density(n_beta_);
+        const double de = energy_ - energy_old_;
+        if (config_.print_level > 0)
+            std::printf("Iter %3d E = %.10f dE = %.4e\n", iter, energy_ + e_nuc, de);
+        if (std::fabs(de) < config_.energy_threshold && iter > 1) {
+            converged = true;
+            break;
+        }
+    }
+    SCFResult res;
+    res.energy_nuclear = e_nuc;
+    res.energy_electronic = energy_;
+    res.energy_total = energy_ + e_nuc;
+    res.C_alpha = res.C_beta = C_;
+    res.P_alpha = Pa_;
+    res.P_beta = Pb_;
+    res.orbital_energies_alpha = res.orbital_energies_beta = eps_;
+    res.converged = converged;
+    res.iterations = iter;
+    res.n_occ_alpha = n_alpha_;
+    res.n_occ_beta = n_beta_;
+    return res;
 }
 
 // ================================================================ CD-RMP2

@@ -1,5 +1,5 @@
 // Cholesky-vector drop-in classes: CholeskyERI, CholeskyRHF, CholeskyUHF,
-// CholeskyRMP2, CholeskyUMP2.  Public names, constructor arguments and
+// CholeskyROHF, CholeskyRMP2, CholeskyUMP2.  Public names, constructor arguments and
 // defaults follow syahrulhidayat/mshqc; the J/K build, the AO->MO transforms
 // and the (ia|jb)/denominator reductions run through libmshqc_hip.so.
 #pragma once
@@ -161,6 +161,44 @@ private:
     void build_fock_cholesky();
     void solve_fock(const Mat& F, Mat& C, Vec& eps) const;
     Mat build_density(const Mat& C, int n_occ) const;
+    double compute_energy() const;
+};
+
+// ------------------------------------------------------------ CD-ROHF
+
+struct CholeskyROHFConfig : public SCFConfig {
+    double cholesky_threshold = 1e-6;
+    bool screen_exchange = true;  // accepted for parity; the reference's ROHF Fock build ignores it
+};
+
+// cholesky_rohf.h / cholesky_rohf.cc: alpha and beta Fock matrices from one
+// shared orbital set, orbitals from the occupation-weighted average
+// (n_a F_a + n_b F_b) / (n_a + n_b), per-spin DIIS below diis_threshold.
+class CholeskyROHF {
+public:
+    CholeskyROHF(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                 int n_alpha, int n_beta, const CholeskyROHFConfig& config = CholeskyROHFConfig());
+    CholeskyROHF(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                 int n_alpha, int n_beta, const CholeskyROHFConfig& config,
+                 const integrals::CholeskyERI& existing_cholesky);
+    SCFResult compute();
+
+private:
+    Molecule mol_;
+    BasisSet basis_;
+    std::shared_ptr<IntegralEngine> integrals_;
+    CholeskyROHFConfig config_;
+    int nbasis_, n_alpha_, n_beta_;
+    integrals::CholeskyERI chol_;
+    Mat S_, H_, X_, C_, Pa_, Pb_, Fa_, Fb_;
+    Vec eps_;
+    double energy_ = 0.0, energy_old_ = 0.0;
+
+    void init_integrals();
+    void initial_guess();
+    void build_fock_cholesky();
+    void solve_fock(const Mat& F);
+    Mat build_density(int n_occ) const;
     double compute_energy() const;
 };
 

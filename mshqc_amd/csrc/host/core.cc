@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <limits>
 #include <fstream>
 #include <sstream>
 
@@ -140,13 +141,19 @@ Vec solve(Mat A, Vec b) {
     const int n = A.rows;
     std::vector<int> colperm(n);
     for (int i = 0; i < n; ++i) colperm[i] = i;
+    int nz = n;  // pivots before the first all-zero corner
+    double maxpivot = 0.0;
     for (int k = 0; k < n; ++k) {
         int pr = k, pc = k;
         double best = -1.0;
-        for (int i = k; i < n; ++i)
-            for (int j = k; j < n; ++j)
+        for (int j = k; j < n; ++j)
+            for (int i = k; i < n; ++i)
                 if (std::fabs(A(i, j)) > best) best = std::fabs(A(i, j)), pr = i, pc = j;
-        if (best <= 0.0) continue;  // singular block: leave as is
+        if (best == 0.0) {
+            nz = k;
+            break;
+        }
+        maxpivot = std::max(maxpivot, best);
         if (pr != k) {
             for (int j = 0; j < n; ++j) std::swap(A(k, j), A(pr, j));
             std::swap(b[k], b[pr]);
@@ -157,16 +164,19 @@ Vec solve(Mat A, Vec b) {
         }
         for (int i = k + 1; i < n; ++i) {
             const double f = A(i, k) / A(k, k);
-            if (f == 0.0) continue;
-            for (int j = k; j < n; ++j) A(i, j) -= f * A(k, j);
+            for (int j = k + 1; j < n; ++j) A(i, j) -= f * A(k, j);
+            A(i, k) = f;
             b[i] -= f * b[k];
         }
     }
+    const double thr = maxpivot * std::numeric_limits<double>::epsilon() * n;
+    int rank = 0;
+    for (int k = 0; k < nz; ++k) rank += std::fabs(A(k, k)) > thr;
     Vec y(n, 0.0);
-    for (int i = n - 1; i >= 0; --i) {
+    for (int i = rank - 1; i >= 0; --i) {
         double s = b[i];
-        for (int j = i + 1; j < n; ++j) s -= A(i, j) * y[j];
-        y[i] = A(i, i) != 0.0 ? s / A(i, i) : 0.0;
+        for (int j = i + 1; j < rank; ++j) s -= A(i, j) * y[j];
+        y[i] = s / A(i, i);
     }
     Vec x(n, 0.0);
     for (int i = 0; i < n; ++i) x[colperm[i]] = y[i];

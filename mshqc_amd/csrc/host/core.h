@@ -45,7 +45,11 @@ Mat right_cols(const Mat& A, int k);
 // Symmetric eigendecomposition (cyclic Jacobi): ascending eigenvalues w,
 // eigenvectors as the columns of V.
 void eigh(const Mat& A, Vec& w, Mat& V);
-// Solve A x = b by Gaussian elimination with full pivoting.
+// Solve A x = b by LU with full pivoting, as the DIIS of the reference does
+// (diis.cc, Eigen fullPivLu().solve): pivots searched column by column,
+// first largest magnitude wins; pivots at or below n * eps * (largest pivot)
+// count as zero and their unknowns are set to 0, so a singular system (e.g.
+// vanishing DIIS error vectors) still returns a solution.
 Vec solve(Mat A, Vec b);
 
 // ----------------------------------------------------------------- molecule

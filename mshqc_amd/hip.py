@@ -36,6 +36,7 @@ _SIGS = {
     "mshqc_hip_memcpy_d2h": [_vp, _vp, _vp, ctypes.c_size_t],
     "mshqc_hip_jk": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _vp, _vp],
     "mshqc_hip_jk_factor": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _vp],
+    "mshqc_hip_jk_rohf": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp],
     "mshqc_hip_half_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp],
     "mshqc_hip_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp],
     "mshqc_hip_rmp2_energy": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D],
@@ -173,6 +174,18 @@ class Context:
                                        self._ptr(Y, (n, m), "Y"), m, self._ptr(J), self._ptr(K))
         self._check(rc, "mshqc_hip_jk_factor")
         return J, K
+
+    def jk_rohf(self, L, C, na, nb):
+        """J[Pa + Pb], Ka, Kb for alpha = C[:, :na], beta = C[:, :nb]."""
+        torch = self._torch
+        Q, n, _ = L.shape
+        ldc = C.shape[1]
+        out = [torch.empty((n, n), dtype=torch.float64, device=L.device) for _ in range(3)]
+        self._follow_stream()
+        rc = lib().mshqc_hip_jk_rohf(self.h, self._ptr(L, name="L"), Q, n, self._ptr(C, (n, ldc), "C"),
+                                     ldc, int(na), int(nb), *[self._ptr(t) for t in out])
+        self._check(rc, "mshqc_hip_jk_rohf")
+        return tuple(out)
 
     def half_transform(self, L, C):
         torch = self._torch

@@ -153,3 +153,56 @@ def test_cd_rhf_tends_to_conventional_rhf(m):
             break
         E = En
     assert abs(res.energy_total - (En + mol.nuclear_repulsion_energy())) < 1e-8
+
+
+def test_cd_rohf_matches_oracle(m):
+    """Li atom (2 alpha, 1 beta), 6-31G, through CholeskyROHF (both
+    constructors) against the oracle's restatement of cholesky_rohf.cc."""
+    mol = m.Molecule(0, 2)
+    mol.add_atom(3, 0.0, 0.0, 0.0)
+    b, eng, chol, L, S, H = setup(m, mol, basis="6-31g")
+    cfg = m.CholeskyROHFConfig()
+    cfg.print_level = 0
+    cfg.energy_threshold = 1e-12
+    cfg.max_iterations = 300
+    assert cfg.cholesky_threshold == 1e-6 and cfg.screen_exchange
+    res = m.CholeskyROHF(mol, b, eng, 2, 1, cfg, chol).compute()
+    ref = scf_ref.cd_rohf(S, H, L, 2, 1, mol.nuclear_repulsion_energy(), max_iterations=300,
+                          energy_threshold=1e-12)
+    print(f"Li CD-ROHF: {res.energy_total!r} oracle {ref['energy']!r} iters {res.iterations}/{ref['iterations']}",
+          flush=True)
+    assert res.converged and ref["converged"]
+    assert abs(res.energy_total - ref["energy"]) < 1e-10
+    assert abs(res.energy_electronic - ref["e_elec"]) < 1e-10
+    assert res.n_occ_alpha == 2 and res.n_occ_beta == 1
+    assert np.array_equal(res.C_alpha, res.C_beta)
+    assert np.abs(res.P_alpha - ref["Pa"]).max() < 1e-8
+    assert np.abs(res.P_beta - ref["Pb"]).max() < 1e-8
+    assert np.array_equal(res.orbital_energies_alpha, res.orbital_energies_beta)
+    # constructor 1 decomposes the integrals itself (same threshold)
+    res1 = m.CholeskyROHF(mol, b, eng, 2, 1, cfg).compute()
+    assert abs(res1.energy_total - res.energy_total) < 1e-10
+
+
+def test_cd_rohf_one_electron_and_closed_shell(m):
+    """H atom (1, 0): the lowest eigenvalue of H, -0.49927840 in cc-pVDZ.
+    H4 chain with n_a = n_b = 2: the CD-RHF energy."""
+    mol = m.Molecule(0, 2)
+    mol.add_atom(1, 0.0, 0.0, 0.0)
+    b, eng, chol, L, S, H = setup(m, mol)
+    cfg = m.CholeskyROHFConfig()
+    cfg.print_level = 0
+    res = m.CholeskyROHF(mol, b, eng, 1, 0, cfg, chol).compute()
+    assert abs(res.energy_total - (-0.49927840)) < 2e-8
+    assert not res.P_beta.any()
+    mol4 = chain(m, 4)
+    b4, eng4, chol4, L4, S4, H4 = setup(m, mol4)
+    cfg.energy_threshold = 1e-12
+    cfg.max_iterations = 300
+    ro = m.CholeskyROHF(mol4, b4, eng4, 2, 2, cfg, chol4).compute()
+    rcfg = m.CholeskyRHFConfig()
+    rcfg.print_level = 0
+    rcfg.energy_threshold = 1e-12
+    rcfg.max_iterations = 300
+    rh = m.CholeskyRHF(mol4, b4, eng4, rcfg, chol4).compute()
+    assert abs(ro.energy_total - rh.energy_total) < 1e-9

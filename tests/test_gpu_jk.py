@@ -56,3 +56,26 @@ def test_jk_factor_matches_general(ctx):
     J, K = ctx.jk_factor(*to_dev(L, Y))
     check_close(to_np(J), J_ref, 1e-12, "J_ref")
     check_close(to_np(K), K_ref, 1e-12, "K_ref")
+
+
+@pytest.mark.parametrize("Q,n,na,nb,extra,seed", [(40, 90, 11, 7, 0, 8), (23, 37, 5, 0, 3, 9),
+                                                  (31, 130, 9, 9, 0, 10), (17, 64, 0, 0, 2, 11),
+                                                  (48, 600, 61, 55, 1, 12), (12, 50, 3, 6, 0, 13)])
+def test_jk_rohf_matches_oracle(ctx, Q, n, na, nb, extra, seed):
+    """Restricted open-shell J/Ka/Kb (occupied-factor form, one half
+    transform for both spins) against the general-density oracle."""
+    rng = np.random.default_rng(seed)
+    L = syn.gen_L(Q, n, rng)
+    m = max(na, nb)
+    C = syn.gen_Cocc(n, max(m + extra, 1), rng)  # ldc = m + extra columns
+    Pa, Pb = C[:, :na] @ C[:, :na].T, C[:, :nb] @ C[:, :nb].T
+    J_ref, K_ref = oracle.jk(L, np.stack([Pa, Pb]))
+    print(f"jk_rohf Q={Q} n={n} na={na} nb={nb} ldc={C.shape[1]}", flush=True)
+    J, Ka, Kb = ctx.jk_rohf(*to_dev(L, C), na, nb)
+    check_close(to_np(J), J_ref, 1e-12, "J_ref")
+    check_close(to_np(Ka), K_ref[0], 1e-12, "Ka_ref")
+    check_close(to_np(Kb), K_ref[1], 1e-12, "Kb_ref")
+    if na == 0:
+        assert not to_np(Ka).any() and not to_np(J).any()
+    if nb == 0:
+        assert not to_np(Kb).any()

@@ -82,3 +82,36 @@ def test_config_defaults_follow_reference(m):
     assert u.diis_max_vectors == 6 and u.max_iterations == 100
     r = m.CholeskyRMP2Config()
     assert r.precision == 0 and r.cholesky_threshold == 1e-6
+
+
+def test_solve_full_pivot(m):
+    """Host LU (DIIS) and the oracle's restatement: numpy's answer on regular
+    systems; the singular DIIS system of vanishing error vectors keeps
+    sum(c) = 1 with the free coefficients at zero."""
+    from oracle.scf_ref import fullpiv_solve
+
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 5, 9):
+        A = rng.standard_normal((n, n))
+        b = rng.standard_normal(n)
+        x_ref = np.linalg.solve(A, b)
+        assert np.allclose(m.linalg_solve(A, b), x_ref, rtol=1e-10, atol=1e-12)
+        assert np.allclose(fullpiv_solve(A, b), x_ref, rtol=1e-10, atol=1e-12)
+    for nv in (2, 3, 5):
+        A = np.zeros((nv + 1, nv + 1))
+        A[:nv, nv] = A[nv, :nv] = -1.0
+        b = np.zeros(nv + 1)
+        b[nv] = -1.0
+        A[:nv, :nv] = 1e-40 * np.eye(nv)  # below the rank threshold
+        for x in (m.linalg_solve(A, b), fullpiv_solve(A, b)):
+            assert x[:nv].sum() == 1.0 and np.count_nonzero(x[:nv]) == 1 and x[nv] == 0.0
+        assert np.array_equal(m.linalg_solve(A, b), fullpiv_solve(A, b))
+    # a DIIS system of well-conditioned error vectors: both agree to rounding
+    E = rng.standard_normal((4, 30))
+    A = np.zeros((5, 5))
+    A[:4, :4] = E @ E.T
+    A[:4, 4] = A[4, :4] = -1.0
+    b = np.zeros(5)
+    b[4] = -1.0
+    assert np.allclose(m.linalg_solve(A, b), fullpiv_solve(A, b), rtol=1e-12, atol=1e-13)
+    assert np.allclose(fullpiv_solve(A, b), np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)

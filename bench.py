@@ -300,6 +300,43 @@ def run_jk(args, torch, hip, syn, w):
     return line
 
 
+def omp2_flops(Q, oa, va, ob, vb, swapped):
+    """FP64 flops the OMP2 amplitude/OPDM call executes: the (ia|kb) GEMMs
+    over all ordered occupied pairs of each spin block (twice for the
+    opposite-spin block when its G_oo^b needs the spin-swapped pass), and the
+    OPDM contractions G_oo = sum T T^T, G_vv = sum T T^T per spin block."""
+    f = 2.0 * Q * ((oa * va) ** 2 + (ob * vb) ** 2 + oa * va * ob * vb * (2 if swapped else 1))
+    for oX, vX in ((oa, va), (ob, vb)):
+        f += 2.0 * oX ** 3 * vX ** 2 + 2.0 * oX ** 2 * vX ** 3
+    f += 2.0 * oa * ob * va * vb * (oa + ob + va + vb)
+    return f
+
+
+def run_omp2(args, torch, hip, syn, w):
+    """One CD-OMP2 iteration's amplitude work: t2 per block, its energy and the
+    four OPDM blocks (the part of cholesky_omp2.cc:94-326 that scales)."""
+    ctx = hip.context(0)
+    Q, oa, va, ob, vb = w["Q"], w["oa"], w["va"], w["ob"], w["vb"]
+    log(f"[bench] generating {args.workload}: Q={Q} oa={oa} va={va} ob={ob} vb={vb}")
+    (Ba, eoa, eva), (Bb, eob, evb) = syn.workload(args.workload)
+    dev = [torch.from_numpy(np.ascontiguousarray(x)).cuda() for x in (Ba, Bb, eoa, eva, eob, evb)]
+    sec, out, prof = timed(torch, ctx, lambda: ctx.omp2_opdm(*dev), args.steps, args.warmup)
+    swapped = 1 << 27 < oa * va * vb * ob
+    flop = omp2_flops(Q, oa, va, ob, vb, swapped)
+    E = out[0]
+    log(f"[bench] omp2_opdm: {sec * 1e3:.1f} ms/step, {flop / sec / 1e12:.1f} TFLOP/s FP64, "
+        f"E = {E}, pair GEMMs {prof[0]}, OPDM GEMMs {prof[1]}")
+    config = {"workload": f"{args.workload}: CD-OMP2 amplitudes, energy and OPDM per iteration", "Q": Q,
+              "oa": oa, "va": va, "ob": ob, "vb": vb, "parallelism": "single GPU"}
+    line = base_line(args, sec, "f64", config, flop)
+    achieved = flop / sec / 1e12
+    line["roofline"] = {"kernel": "FP64 MFMA GEMMs (pair integrals + OPDM contractions)", "bound": "mfma",
+                        "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s", "frac": achieved / 78.6,
+                        "traffic": None}
+    line["energy_components"] = list(E)
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,6 +365,8 @@ def main():
     w = syn.WORKLOADS[args.workload]
     if w["kind"] in ("rmp2", "ump2"):
         line = run_mp2(args, torch, hip, syn, w)
+    elif w["kind"] == "omp2":
+        line = run_omp2(args, torch, hip, syn, w)
     else:
         line = run_jk(args, torch, hip, syn, w)
     print(json.dumps(line), flush=True)

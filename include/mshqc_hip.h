@@ -62,8 +62,9 @@ int mshqc_hip_set_stream(mshqc_hip_ctx* ctx, void* stream);
 /* Device time, in ms, of the last compute call (HIP events around its kernels). */
 int mshqc_hip_last_ms(const mshqc_hip_ctx* ctx, float* ms);
 /* Cap, in doubles, of the per-occupied-block amplitude workspace used by the
- * gradient calls (default 2^27 = 1 GiB).  Smaller caps mean more, smaller
- * occupied blocks; results do not depend on it beyond rounding. */
+ * gradient calls and by mshqc_hip_omp2_opdm (default 2^27 = 1 GiB).  Smaller
+ * caps mean more, smaller occupied blocks; results do not depend on it beyond
+ * rounding. */
 int mshqc_hip_set_zblock(mshqc_hip_ctx* ctx, size_t elems);
 /* Per-kernel-class timing (HIP events around each launch, on the stream the
  * kernel runs on).  kind: 0 = pair-block (ia|jb) kernels, 1 = GEMM kernels
@@ -193,6 +194,31 @@ int mshqc_hip_ump2_grad(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
                         double* dEdBa, double* dEdBb,
                         double* dE_deps_oa, double* dE_deps_va,
                         double* dE_deps_ob, double* dE_deps_vb);
+
+/* ---------------------------------------------------------------- OMP2 */
+
+/*
+ * One CD-OMP2 iteration's amplitude work, replacing
+ * CholeskyOMP2::compute_t2_amplitudes, compute_mp2_energy_from_t2 and
+ * build_opdm_alpha/beta (src/mp2/cholesky_omp2.cc:94-326).  With
+ * g_ijab = sum_Q B^Q_ia B^Q_jb and D the orbital-energy denominator
+ * (|D| < 1e-12 gives t = 0):
+ *   t_aa(ijab) = (g_ijab - g_ijba)/D (i != j, a != b; else 0), t_bb alike,
+ *   t_ab(ijab) = g_ijab/D  (i alpha, j beta)
+ *   E[0] = sum_{i<j, a<b} t_aa^2 D,  E[1] likewise for beta,
+ *   E[2] = sum t_ab^2 D                                   (E: HOST, 3)
+ *   Goo_a(i,j) = -(1/2 sum_kab t_aa(ikab) t_aa(jkab) + sum_kab t_ab(ikab) t_ab(jkab))
+ *   Gvv_a(a,b) =  1/2 sum_ijc t_aa(ijac) t_aa(ijbc) + sum_ijc t_ab(ijac) t_ab(ijbc)
+ *   Goo_b, Gvv_b with the roles of the spins exchanged in t_ab.
+ * G blocks are DEVICE arrays [o][o] and [v][v].  The amplitudes are formed
+ * block by block (at most the mshqc_hip_set_zblock cap per block) and never
+ * stored whole.
+ */
+int mshqc_hip_omp2_opdm(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
+                        int Q, int oa, int va, int ob, int vb,
+                        const double* eps_oa, const double* eps_va,
+                        const double* eps_ob, const double* eps_vb, double* E,
+                        double* Goo_a, double* Gvv_a, double* Goo_b, double* Gvv_b);
 
 #ifdef __cplusplus
 }

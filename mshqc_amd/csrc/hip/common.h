@@ -63,6 +63,8 @@ enum WsSlot {
     WS_TMP2 = 7,
     WS_ZS = 8,     // split-BF16 slices of the amplitude block
     WS_ZTS = 9,    // ... and of its transpose
+    WS_OMP2_V = 10,  // OMP2 pair-integral block (ia|kb)
+    WS_OMP2_T = 11,  // OMP2 amplitude block
 };
 
 // Sets ctx->last_error and returns the status code.
@@ -127,6 +129,10 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g);
 // Deterministic single-workgroup sum of x[0..n) into *out (device), using a
 // fixed per-thread stride and a fixed tree; Neumaier compensation per thread.
 int reduce_sum(mshqc_hip_ctx* ctx, const double* x, long n, double* out, bool accumulate);
+
+// x[0..count) = 0 with a fill kernel on the ctx stream (no memset node, so
+// it stays correct inside captured graphs).
+int fill_zero(mshqc_hip_ctx* ctx, double* x, long count);
 
 // Number of bytes in a mebibyte, for messages.
 static constexpr double kMiB = 1024.0 * 1024.0;

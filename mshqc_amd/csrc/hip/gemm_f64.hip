@@ -203,6 +203,18 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
     return MSHQC_HIP_OK;
 }
 
+static __global__ void fill_zero_kernel(double* __restrict__ x, long count) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) x[k] = 0.0;
+}
+
+int fill_zero(mshqc_hip_ctx* ctx, double* x, long count) {
+    if (count <= 0) return MSHQC_HIP_OK;
+    fill_zero_kernel<<<(unsigned)((count + 255) / 256), 256, 0, ctx->stream>>>(x, count);
+    LAUNCH_CHECK(ctx);
+    return MSHQC_HIP_OK;
+}
+
 int reduce_sum(mshqc_hip_ctx* ctx, const double* x, long n, double* out, bool accumulate) {
     reduce_sum_kernel<<<1, 1024, 0, ctx->stream>>>(x, n, out, accumulate ? 1 : 0);
     LAUNCH_CHECK(ctx);

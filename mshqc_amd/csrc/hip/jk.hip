@@ -97,17 +97,6 @@ int j_build(mshqc_hip_ctx* ctx, const double* L, int Q, long nn, const double* P
     return MSHQC_HIP_OK;
 }
 
-__global__ void fill_zero_kernel(double* __restrict__ x, long count) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) x[k] = 0.0;
-}
-
-int fill_zero(mshqc_hip_ctx* ctx, double* x, long count) {
-    fill_zero_kernel<<<(unsigned)((count + 255) / 256), 256, 0, ctx->stream>>>(x, count);
-    LAUNCH_CHECK(ctx);
-    return MSHQC_HIP_OK;
-}
-
 // Largest Q chunk whose n x m temporaries fit in `cap` doubles.
 int q_chunk(int Q, long per_q, size_t cap) {
     long c = (long)(cap / (size_t)std::max(per_q, 1L));

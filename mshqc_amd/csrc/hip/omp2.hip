@@ -1,0 +1,282 @@
+// CD-OMP2 amplitudes, amplitude energy and one-particle density blocks
+// (cholesky_omp2.cc:94-326) from the MO Cholesky vectors B^Q_ia of each spin.
+//
+// The reference stores the three amplitude tensors t_aa, t_bb, t_ab whole.
+// Here each spin block (X, Y) is walked in blocks of its second occupied
+// index k:
+//   V[(i a)][(k b)] = (ia|kb) = sum_Q B_X^Q_ia B_Y^Q_kb     FP64 MFMA GEMM
+//   T[k][i][a][b]   = amplitude (antisymmetrised for X = Y)  elementwise
+//   G_oo^X += s_o sum_k T_k T_k^T,  G_vv^X += s_v sum_{k,i} T_ki T_ki^T
+//   (opposite spin also G_vv^Y += sum_{k,i} T_ki^T T_ki and G_oo^Y)
+// so memory is bounded by two blocks, not by o^2 v^2 per spin pair.
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+
+namespace {
+
+constexpr int AMP_THREADS = 256;
+constexpr int AMP_MAX_GROUPS = 4096;
+
+// Amplitudes of one k block.  V row (i, a), column (kk, b) holds (ia|kb),
+// k = k0 + kk.  same != 0: t = ((ia|kb) - (ib|ka)) / D, zero when i == k or
+// a == b (cholesky_omp2.cc:115-137); else t = (ia|kb) / D (:161-180).
+// |D| < 1e-12 leaves t = 0.  part[g] = sum of t^2 D over workgroup g's
+// elements (fixed grid-stride assignment, so the sum is reproducible).
+__global__ __launch_bounds__(AMP_THREADS) void omp2_amp_kernel(
+    const double* __restrict__ V, long ldv, int oX, int vX, int vY, int k0, int nk,
+    const double* __restrict__ eoX, const double* __restrict__ evX, const double* __restrict__ eoY,
+    const double* __restrict__ evY, int same, double* __restrict__ T, double* __restrict__ part) {
+    const long total = (long)nk * oX * vX * vY;
+    double e = 0.0;
+    for (long t = (long)blockIdx.x * AMP_THREADS + threadIdx.x; t < total;
+         t += (long)gridDim.x * AMP_THREADS) {
+        const int b = (int)(t % vY);
+        long r = t / vY;
+        const int a = (int)(r % vX);
+        r /= vX;
+        const int i = (int)(r % oX);
+        const int kk = (int)(r / oX);
+        const int k = k0 + kk;
+        const double D = eoX[i] + eoY[k] - evX[a] - evY[b];
+        const double g = V[((long)i * vX + a) * ldv + (long)kk * vY + b];
+        double val = 0.0;
+        if (fabs(D) >= 1e-12) {
+            if (!same) {
+                val = g / D;
+            } else if (i != k && a != b) {
+                val = (g - V[((long)i * vX + b) * ldv + (long)kk * vY + a]) / D;
+            }
+        }
+        T[t] = val;
+        e += val * val * D;
+    }
+    for (int m = 32; m > 0; m >>= 1) e += __shfl_xor(e, m, 64);
+    __shared__ double sh[AMP_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+struct SpinPair {
+    const double* BX;  // [Q][oX][vX]
+    const double* BY;  // [Q][oY][vY]
+    int oX, vX, oY, vY;
+    const double *eoX, *evX, *eoY, *evY;
+    bool same;
+};
+
+struct Want {
+    double* GooX = nullptr;  // [oX][oX], scaled by -1/2 (same) or -1
+    double* GvvX = nullptr;  // [vX][vX], scaled by 1/2 (same) or 1
+    double* GvvY = nullptr;  // [vY][vY] (opposite spin only)
+    double* GooY = nullptr;  // [oY][oY] (opposite spin, single k block only)
+    double* e_slot = nullptr;
+};
+
+// Batch layers of a batch-reduced GEMM: at most 256, and partials
+// ([splits][M][N]) within 2^25 doubles.
+int red_splits(int batch, long M, long N) {
+    const long cap = 1L << 25;
+    const long s = std::min<long>(batch, std::max<long>(1, cap / std::max<long>(M * N, 1)));
+    return (int)std::min<long>(s, 256);
+}
+
+// Largest k block whose V and T blocks (oX vX vY doubles per k) fit `cap`.
+int k_block(const SpinPair& s, size_t cap) {
+    const size_t per_k = (size_t)s.oX * s.vX * s.vY;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)s.oY, cap / std::max<size_t>(per_k, 1)));
+}
+
+int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int kb) {
+    const long per_k = (long)s.oX * s.vX * s.vY;
+    double* V = static_cast<double*>(workspace(ctx, WS_OMP2_V, sizeof(double) * (size_t)kb * per_k));
+    double* T = static_cast<double*>(workspace(ctx, WS_OMP2_T, sizeof(double) * (size_t)kb * per_k));
+    double* part = static_cast<double*>(workspace(ctx, WS_PART, sizeof(double) * AMP_MAX_GROUPS));
+    if (!V || !T || !part) return MSHQC_HIP_ERR_ALLOC;
+    const double so = s.same ? -0.5 : -1.0, sv = s.same ? 0.5 : 1.0;
+    int rc;
+    for (int k0 = 0; k0 < s.oY; k0 += kb) {
+        const int nk = std::min(kb, s.oY - k0);
+        const long ldv = (long)nk * s.vY;
+        // V = B_X^T B_Y[:, k0:k0+nk, :] over Q
+        GemmF64 gv;
+        gv.A = s.BX;
+        gv.lda = (long)s.oX * s.vX;
+        gv.transA = true;
+        gv.X = s.BY + (long)k0 * s.vY;
+        gv.ldx = (long)s.oY * s.vY;
+        gv.C = V;
+        gv.ldc = ldv;
+        gv.M = s.oX * s.vX;
+        gv.N = (int)ldv;
+        gv.K = Q;
+        { ProfScope ps(ctx, PROF_PAIR); rc = gemm_f64(ctx, gv); }
+        if (rc) return rc;
+        const long total = (long)nk * per_k;
+        const int groups = (int)std::min<long>(AMP_MAX_GROUPS, (total + AMP_THREADS - 1) / AMP_THREADS);
+        omp2_amp_kernel<<<groups, AMP_THREADS, 0, ctx->stream>>>(V, ldv, s.oX, s.vX, s.vY, k0, nk, s.eoX,
+                                                                 s.evX, s.eoY, s.evY, s.same ? 1 : 0, T,
+                                                                 part);
+        LAUNCH_CHECK(ctx);
+        if (w.e_slot && (rc = reduce_sum(ctx, part, groups, w.e_slot, true))) return rc;
+        const long vv = (long)s.vX * s.vY;
+        if (w.GooX) {  // sum_k T_k T_k^T, T_k = [oX][vX vY]
+            GemmF64 g;
+            g.A = T;
+            g.lda = vv;
+            g.sA = per_k;
+            g.X = T;
+            g.ldx = vv;
+            g.sX = per_k;
+            g.transX = true;
+            g.C = w.GooX;
+            g.ldc = s.oX;
+            g.M = g.N = s.oX;
+            g.K = (int)vv;
+            g.batch = nk;
+            g.reduce_batch = true;
+            g.splits = red_splits(nk, s.oX, s.oX);
+            g.alpha = so;
+            g.accumulate = true;
+            { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
+            if (rc) return rc;
+        }
+        if (w.GvvX) {  // sum_{k,i} T_ki T_ki^T, T_ki = [vX][vY]
+            GemmF64 g;
+            g.A = T;
+            g.lda = s.vY;
+            g.sA = vv;
+            g.X = T;
+            g.ldx = s.vY;
+            g.sX = vv;
+            g.transX = true;
+            g.C = w.GvvX;
+            g.ldc = s.vX;
+            g.M = g.N = s.vX;
+            g.K = s.vY;
+            g.batch = nk * s.oX;
+            g.reduce_batch = true;
+            g.splits = red_splits(nk * s.oX, s.vX, s.vX);
+            g.alpha = sv;
+            g.accumulate = true;
+            { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
+            if (rc) return rc;
+        }
+        if (w.GvvY) {  // sum_{k,i} T_ki^T T_ki
+            GemmF64 g;
+            g.A = T;
+            g.lda = s.vY;
+            g.sA = vv;
+            g.transA = true;
+            g.X = T;
+            g.ldx = s.vY;
+            g.sX = vv;
+            g.C = w.GvvY;
+            g.ldc = s.vY;
+            g.M = g.N = s.vY;
+            g.K = s.vX;
+            g.batch = nk * s.oX;
+            g.reduce_batch = true;
+            g.splits = red_splits(nk * s.oX, s.vY, s.vY);
+            g.alpha = 1.0;
+            g.accumulate = true;
+            { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
+            if (rc) return rc;
+        }
+        if (w.GooY) {  // one block holds every k: -sum_i T_(.i) T_(.i)^T over (a, b)
+            GemmF64 g;
+            g.A = T;
+            g.lda = per_k;
+            g.sA = vv;
+            g.X = T;
+            g.ldx = per_k;
+            g.sX = vv;
+            g.transX = true;
+            g.C = w.GooY;
+            g.ldc = s.oY;
+            g.M = g.N = nk;
+            g.K = (int)vv;
+            g.batch = s.oX;
+            g.reduce_batch = true;
+            g.splits = red_splits(s.oX, nk, nk);
+            g.alpha = -1.0;
+            g.accumulate = true;
+            { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
+            if (rc) return rc;
+        }
+    }
+    return MSHQC_HIP_OK;
+}
+
+__global__ void omp2_energy_out_kernel(const double* __restrict__ slots, double* __restrict__ out) {
+    if (threadIdx.x == 0) {
+        out[0] = 0.25 * slots[0];
+        out[1] = 0.25 * slots[1];
+        out[2] = slots[2];
+    }
+}
+
+}  // namespace
+
+extern "C" int mshqc_hip_omp2_opdm(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb, int Q, int oa,
+                                   int va, int ob, int vb, const double* eps_oa, const double* eps_va,
+                                   const double* eps_ob, const double* eps_vb, double* E3,
+                                   double* Goo_a, double* Gvv_a, double* Goo_b, double* Gvv_b) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, E3 != nullptr, "omp2_opdm: null energy pointer");
+    CHECK_ARG(ctx, Q > 0 && oa >= 0 && va >= 0 && ob >= 0 && vb >= 0, "omp2_opdm: bad dimensions");
+    CHECK_ARG(ctx, (oa == 0 || (Goo_a && eps_oa)) && (va == 0 || (Gvv_a && eps_va)) &&
+                       (ob == 0 || (Goo_b && eps_ob)) && (vb == 0 || (Gvv_b && eps_vb)),
+              "omp2_opdm: null output or orbital-energy pointer");
+    CHECK_ARG(ctx, (oa * va == 0 || Ba) && (ob * vb == 0 || Bb), "omp2_opdm: null B");
+    hipEventRecord(ctx->ev0, ctx->stream);
+    double* slots = static_cast<double*>(workspace(ctx, WS_MISC, sizeof(double) * 8));
+    if (!slots) return MSHQC_HIP_ERR_ALLOC;
+    if ((rc = fill_zero(ctx, slots, 8))) return rc;
+    if ((rc = fill_zero(ctx, Goo_a, (long)oa * oa)) || (rc = fill_zero(ctx, Gvv_a, (long)va * va)) ||
+        (rc = fill_zero(ctx, Goo_b, (long)ob * ob)) || (rc = fill_zero(ctx, Gvv_b, (long)vb * vb)))
+        return rc;
+    const size_t cap = ctx->zblock_elems;
+    // same spin: alpha-alpha, beta-beta (cholesky_omp2.cc:115-158, 232-326)
+    const SpinPair aa{Ba, Ba, oa, va, oa, va, eps_oa, eps_va, eps_oa, eps_va, true};
+    const SpinPair bb{Bb, Bb, ob, vb, ob, vb, eps_ob, eps_vb, eps_ob, eps_vb, true};
+    for (int s = 0; s < 2; ++s) {
+        const SpinPair& sp = s == 0 ? aa : bb;
+        if (sp.oX < 2 || sp.vX < 2) continue;  // no i < j, a < b pairs
+        Want w;
+        w.GooX = s == 0 ? Goo_a : Goo_b;
+        w.GvvX = s == 0 ? Gvv_a : Gvv_b;
+        w.e_slot = slots + s;
+        if ((rc = spin_pair(ctx, Q, sp, w, k_block(sp, cap)))) return rc;
+    }
+    // opposite spin (cholesky_omp2.cc:161-180 and the t_ab terms of the OPDM)
+    if (oa > 0 && va > 0 && ob > 0 && vb > 0) {
+        const SpinPair ab{Ba, Bb, oa, va, ob, vb, eps_oa, eps_va, eps_ob, eps_vb, false};
+        const int kb = k_block(ab, cap);
+        Want w;
+        w.GooX = Goo_a;
+        w.GvvX = Gvv_a;
+        w.GvvY = Gvv_b;
+        w.GooY = kb >= ob ? Goo_b : nullptr;
+        w.e_slot = slots + 2;
+        if ((rc = spin_pair(ctx, Q, ab, w, kb))) return rc;
+        if (kb < ob) {
+            // G_oo^b needs every beta k at once: walk t_ab again with the
+            // spins swapped, blocked over the alpha index
+            const SpinPair ba{Bb, Ba, ob, vb, oa, va, eps_ob, eps_vb, eps_oa, eps_va, false};
+            Want wb;
+            wb.GooX = Goo_b;
+            if ((rc = spin_pair(ctx, Q, ba, wb, k_block(ba, cap)))) return rc;
+        }
+    }
+    omp2_energy_out_kernel<<<1, 64, 0, ctx->stream>>>(slots, slots + 4);
+    LAUNCH_CHECK(ctx);
+    HIP_TRY(ctx, hipMemcpyAsync(E3, slots + 4, sizeof(double) * 3, hipMemcpyDeviceToHost, ctx->stream));
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if ((rc = ctx_finish(ctx))) return rc;
+    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+    return MSHQC_HIP_OK;
+}

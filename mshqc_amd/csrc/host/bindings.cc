@@ -294,6 +294,45 @@ PYBIND11_MODULE(_mshqc, m) {
              py::arg("config"), py::arg("existing_cholesky"))
         .def("compute", &CholeskyROHF::compute);
 
+    // ---------------------------------------------------------- OMP2
+    // bindings.cc:311-331 (OMP2Result) and 495-519 of the reference
+    py::class_<OMP2Result> omp2r(m, "OMP2Result");
+    omp2r.def(py::init<>())
+        .def_readwrite("energy_scf", &OMP2Result::energy_scf)
+        .def_readwrite("energy_mp2_ss", &OMP2Result::energy_mp2_ss)
+        .def_readwrite("energy_mp2_os", &OMP2Result::energy_mp2_os)
+        .def_readwrite("energy_mp2_corr", &OMP2Result::energy_mp2_corr)
+        .def_readwrite("energy_total", &OMP2Result::energy_total)
+        .def_readwrite("converged", &OMP2Result::converged)
+        .def_readwrite("iterations", &OMP2Result::iterations)
+        .def_readwrite("n_occ_alpha", &OMP2Result::n_occ_alpha)
+        .def_readwrite("n_occ_beta", &OMP2Result::n_occ_beta)
+        .def_readwrite("n_virt_alpha", &OMP2Result::n_virt_alpha)
+        .def_readwrite("n_virt_beta", &OMP2Result::n_virt_beta);
+    mat_prop(omp2r, "C_alpha", &OMP2Result::C_alpha);
+    mat_prop(omp2r, "C_beta", &OMP2Result::C_beta);
+    vec_prop(omp2r, "orbital_energies_alpha", &OMP2Result::orbital_energies_alpha);
+    vec_prop(omp2r, "orbital_energies_beta", &OMP2Result::orbital_energies_beta);
+
+    py::class_<CholeskyOMP2Config>(m, "CholeskyOMP2Config")
+        .def(py::init<>())
+        .def_readwrite("max_iterations", &CholeskyOMP2Config::max_iterations)
+        .def_readwrite("energy_threshold", &CholeskyOMP2Config::energy_threshold)
+        .def_readwrite("gradient_threshold", &CholeskyOMP2Config::gradient_threshold)
+        .def_readwrite("cholesky_threshold", &CholeskyOMP2Config::cholesky_threshold)
+        .def_readwrite("print_level", &CholeskyOMP2Config::print_level);
+
+    py::class_<CholeskyOMP2>(m, "CholeskyOMP2")
+        .def(py::init<const Molecule&, const BasisSet&, std::shared_ptr<IntegralEngine>, const SCFResult&,
+                      const CholeskyOMP2Config&>(),
+             py::arg("molecule"), py::arg("basis"), py::arg("integrals"), py::arg("scf_guess"),
+             py::arg("config") = CholeskyOMP2Config())
+        .def(py::init<const Molecule&, const BasisSet&, std::shared_ptr<IntegralEngine>, const SCFResult&,
+                      const CholeskyOMP2Config&, const CholeskyERI&>(),
+             py::arg("molecule"), py::arg("basis"), py::arg("integrals"), py::arg("scf_guess"), py::arg("config"),
+             py::arg("existing_cholesky"), py::keep_alive<1, 7>())
+        .def("compute", &CholeskyOMP2::compute);
+
     // ----------------------------------------------------------- MP2
     py::class_<CholeskyRMP2Config>(m, "CholeskyRMP2Config")
         .def(py::init<>())
@@ -362,6 +401,7 @@ PYBIND11_MODULE(_mshqc, m) {
         eigh(np_to_mat(A), w, V);
         return py::make_tuple(vec_to_np(w), mat_to_np(V));
     });
+    m.def("linalg_expm", [](const arr& A) { return mat_to_np(expm(np_to_mat(A))); });
     m.def("linalg_solve", [](const arr& A, const arr& b) {
         const Mat M = np_to_mat(A);
         auto bb = b.unchecked<1>();

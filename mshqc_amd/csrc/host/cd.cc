@@ -735,4 +735,170 @@ CholeskyUMP2Result CholeskyUMP2::compute() {
     return r;
 }
 
+// ================================================================ CD-OMP2
+
+CholeskyOMP2::CholeskyOMP2(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                           const SCFResult& scf_guess, const CholeskyOMP2Config& config)
+    : mol_(mol), basis_(basis), integrals_(std::move(integrals)), scf_(scf_guess), config_(config) {
+    // cholesky_omp2.cc:18-41: decompose with the configured threshold
+    nbf_ = scf_.C_alpha.rows;
+    na_ = scf_.n_occ_alpha;
+    nb_ = scf_.n_occ_beta;
+    va_ = nbf_ - na_;
+    vb_ = nbf_ - nb_;
+    if (config_.print_level > 0)
+        std::printf("[CholeskyOMP2] Decomposing integrals (Thresh=%g)...\n", config_.cholesky_threshold);
+    internal_ = std::make_unique<integrals::CholeskyERI>(config_.cholesky_threshold);
+    internal_->decompose(integrals_->compute_eri());
+    chol_ = internal_.get();
+}
+
+CholeskyOMP2::CholeskyOMP2(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                           const SCFResult& scf_guess, const CholeskyOMP2Config& config,
+                           const integrals::CholeskyERI& cholesky_vectors)
+    : mol_(mol), basis_(basis), integrals_(std::move(integrals)), scf_(scf_guess), config_(config),
+      chol_(&cholesky_vectors) {
+    // cholesky_omp2.cc:46-66; the reference exits the process on empty
+    // vectors, this throws
+    nbf_ = scf_.C_alpha.rows;
+    na_ = scf_.n_occ_alpha;
+    nb_ = scf_.n_occ_beta;
+    va_ = nbf_ - na_;
+    vb_ = nbf_ - nb_;
+    if (chol_->n_vectors() == 0)
+        throw std::runtime_error("CholeskyOMP2 initialized with empty reused vectors");
+}
+
+void CholeskyOMP2::rotate(const Mat& w, const Mat& F_mo, int no, int nv, Mat& C, Mat& P) const {
+    // cholesky_omp2.cc:375-425: kappa_ai = 0.5 w_ai / (F_aa - F_ii + 0.2),
+    // denominators clamped to |.| >= 1e-2, C <- C exp(-kappa)
+    const double damping = 0.5, shift = 0.2;
+    Mat kappa(nbf_, nbf_);
+    for (int a = 0; a < nv; ++a)
+        for (int i = 0; i < no; ++i) {
+            double diff = F_mo(no + a, no + a) - F_mo(i, i) + shift;
+            if (std::fabs(diff) < 1e-2) diff = diff >= 0 ? 1e-2 : -1e-2;
+            const double val = (w(a, i) / diff) * damping;
+            kappa(no + a, i) = val;
+            kappa(i, no + a) = -val;
+        }
+    C = matmul(C, expm(scale(kappa, -1.0)));
+    Mat Co = left_cols(C, no);
+    P = matmul_nt(Co, Co);
+}
+
+OMP2Result CholeskyOMP2::compute() {
+    // cholesky_omp2.cc:427-513
+    const int n = nbf_, Q = chol_->n_vectors();
+    if (chol_->n_basis() != n) throw std::runtime_error("CholeskyOMP2: Cholesky vectors do not match the basis");
+    if (config_.print_level > 0)
+        std::printf("\n========================================\n     Cholesky Orbital-Optimized MP2\n"
+                    "     Vectors: %d\n========================================\n",
+                    Q);
+    const Mat H = add(integrals_->compute_kinetic(), integrals_->compute_nuclear());
+    const Vec &ea = scf_.orbital_energies_alpha, &eb = scf_.orbital_energies_beta;
+    // the denominators keep the SCF orbital energies on every iteration
+    auto deoa = to_device(Vec(ea.begin(), ea.begin() + na_));
+    auto deva = to_device(Vec(ea.begin() + na_, ea.end()));
+    auto deob = to_device(Vec(eb.begin(), eb.begin() + nb_));
+    auto devb = to_device(Vec(eb.begin() + nb_, eb.end()));
+    device::Buffer dGoa((size_t)std::max(na_ * na_, 1)), dGva((size_t)std::max(va_ * va_, 1)),
+        dGob((size_t)std::max(nb_ * nb_, 1)), dGvb((size_t)std::max(vb_ * vb_, 1));
+    device::Buffer dJ((size_t)n * n), dK((size_t)2 * n * n);
+    double e_total_prev = scf_.energy_total, e_mp2 = 0.0, e_ss = 0.0, e_os = 0.0;
+    bool converged = false;
+    int iter = 0;
+    while (iter < config_.max_iterations) {
+        ++iter;
+        // amplitudes, their energy and the OPDM blocks (:94-326)
+        std::unique_ptr<device::Buffer> Ba, Bb;
+        if (na_ > 0 && va_ > 0) Ba = transform_device(*chol_, scf_.C_alpha, na_);
+        const bool same_orbitals = na_ == nb_ && scf_.C_beta.a == scf_.C_alpha.a;
+        if (nb_ > 0 && vb_ > 0 && !same_orbitals) Bb = transform_device(*chol_, scf_.C_beta, nb_);
+        const double* pBb = same_orbitals ? (Ba ? Ba->get() : nullptr) : (Bb ? Bb->get() : nullptr);
+        double E[3] = {0, 0, 0};
+        device::check(mshqc_hip_omp2_opdm(device::ctx(), Ba ? Ba->get() : nullptr, pBb, Q, na_, va_, nb_, vb_,
+                                          deoa->get(), deva->get(), deob->get(), devb->get(), E, dGoa.get(),
+                                          dGva.get(), dGob.get(), dGvb.get()),
+                      "mshqc_hip_omp2_opdm");
+        e_ss = E[0] + E[1];
+        e_os = E[2];
+        e_mp2 = (E[0] + E[1]) + E[2];
+        const double e_total_curr = scf_.energy_total + e_mp2;
+        // P_tot = P + C G C^T per spin (:451-459)
+        Mat Pt[2];
+        for (int s = 0; s < 2; ++s) {
+            const int no = s == 0 ? na_ : nb_, nv = s == 0 ? va_ : vb_;
+            const Mat& C = s == 0 ? scf_.C_alpha : scf_.C_beta;
+            Mat Goo(no, no), Gvv(nv, nv);
+            if (no > 0) (s == 0 ? dGoa : dGob).download(Goo.data(), Goo.size());
+            if (nv > 0) (s == 0 ? dGva : dGvb).download(Gvv.data(), Gvv.size());
+            Mat G(n, n);
+            for (int i = 0; i < no; ++i)
+                for (int j = 0; j < no; ++j) G(i, j) = Goo(i, j);
+            for (int a = 0; a < nv; ++a)
+                for (int b = 0; b < nv; ++b) G(no + a, no + b) = Gvv(a, b);
+            Pt[s] = add(s == 0 ? scf_.P_alpha : scf_.P_beta, matmul_nt(matmul(C, G), C));
+        }
+        // generalised Fock per spin: H + J[Pt_a + Pt_b] - K[Pt_s], in the MO
+        // basis (:331-369, :462-467; one J for both spins)
+        Mat P2(2 * n, n);
+        std::copy(Pt[0].a.begin(), Pt[0].a.end(), P2.a.begin());
+        std::copy(Pt[1].a.begin(), Pt[1].a.end(), P2.a.begin() + (size_t)n * n);
+        auto dP = to_device(P2);
+        device::check(mshqc_hip_jk(device::ctx(), chol_->device_vectors(), Q, n, dP->get(), 2, 0, dJ.get(), dK.get()),
+                      "mshqc_hip_jk");
+        Mat J(n, n), K(2 * n, n);
+        dJ.download(J.data(), J.size());
+        dK.download(K.data(), K.size());
+        Mat F_mo[2], w[2];
+        double g2 = 0.0;
+        for (int s = 0; s < 2; ++s) {
+            Mat Ks(n, n);
+            std::copy(K.a.begin() + (size_t)s * n * n, K.a.begin() + (size_t)(s + 1) * n * n, Ks.a.begin());
+            const Mat F_ao = add(add(H, J), Ks, -1.0);
+            const Mat& C = s == 0 ? scf_.C_alpha : scf_.C_beta;
+            F_mo[s] = matmul(matmul_tn(C, F_ao), C);
+            const int no = s == 0 ? na_ : nb_, nv = s == 0 ? va_ : vb_;
+            w[s] = Mat(nv, no);
+            for (int a = 0; a < nv; ++a)
+                for (int i = 0; i < no; ++i) w[s](a, i) = 2.0 * F_mo[s](no + a, i);
+            g2 += dot(w[s], w[s]);
+        }
+        const double g_norm = std::sqrt(g2);
+        if (config_.print_level > 0)
+            std::printf("%4d    %.8f    %.8f    %.2e\n", iter, e_total_curr, e_mp2, g_norm);
+        const double de = std::fabs(e_total_curr - e_total_prev);
+        if (g_norm < config_.gradient_threshold && de < config_.energy_threshold) {
+            converged = true;
+            break;
+        }
+        rotate(w[0], F_mo[0], na_, va_, scf_.C_alpha, scf_.P_alpha);
+        if (na_ != nb_) {
+            rotate(w[1], F_mo[1], nb_, vb_, scf_.C_beta, scf_.P_beta);
+        } else {
+            scf_.C_beta = scf_.C_alpha;
+            scf_.P_beta = scf_.P_alpha;
+        }
+        e_total_prev = e_total_curr;
+    }
+    OMP2Result res;
+    res.energy_scf = scf_.energy_total;
+    res.energy_total = e_total_prev;  // the reference keeps the previous iteration's total on exit
+    res.energy_mp2_corr = e_mp2;
+    res.energy_mp2_ss = e_ss;
+    res.energy_mp2_os = e_os;
+    res.converged = converged;
+    res.iterations = iter;
+    res.n_occ_alpha = na_;
+    res.n_occ_beta = nb_;
+    res.n_virt_alpha = va_;
+    res.n_virt_beta = vb_;
+    res.C_alpha = scf_.C_alpha;
+    res.C_beta = scf_.C_beta;
+    res.orbital_energies_alpha = scf_.orbital_energies_alpha;
+    res.orbital_energies_beta = scf_.orbital_energies_beta;
+    return res;
+}
+
 }  // namespace mshqc

@@ -1,5 +1,5 @@
 // Cholesky-vector drop-in classes: CholeskyERI, CholeskyRHF, CholeskyUHF,
-// CholeskyROHF, CholeskyRMP2, CholeskyUMP2.  Public names, constructor arguments and
+// CholeskyROHF, CholeskyRMP2, CholeskyUMP2, CholeskyOMP2.  Public names, constructor arguments and
 // defaults follow syahrulhidayat/mshqc; the J/K build, the AO->MO transforms
 // and the (ia|jb)/denominator reductions run through libmshqc_hip.so.
 #pragma once
@@ -274,6 +274,55 @@ private:
     CholeskyUMP2Config config_;
     integrals::CholeskyERI chol_;  // shares the device copy with the source
     int nbf_, na_, nb_, va_, vb_;
+};
+
+// ------------------------------------------------------------- CD-OMP2
+
+// mp2.h:52-72 of the reference.  energy_mp2_ss / energy_mp2_os, which the
+// reference's CholeskyOMP2 leaves unset, carry the same-spin and
+// opposite-spin parts of the last iteration's energy here.
+struct OMP2Result {
+    double energy_scf = 0.0, energy_mp2_ss = 0.0, energy_mp2_os = 0.0, energy_mp2_corr = 0.0,
+           energy_total = 0.0;
+    int n_occ_alpha = 0, n_occ_beta = 0, n_virt_alpha = 0, n_virt_beta = 0;
+    Vec orbital_energies_alpha, orbital_energies_beta;
+    Mat C_alpha, C_beta;
+    bool converged = false;
+    int iterations = 0;
+};
+
+struct CholeskyOMP2Config {
+    int max_iterations = 50;
+    double energy_threshold = 1e-6;
+    double gradient_threshold = 1e-5;
+    double cholesky_threshold = 1e-6;
+    int print_level = 1;
+};
+
+// cholesky_omp2.h / cholesky_omp2.cc: MP2 amplitudes and one-particle density
+// from the rotated orbitals each iteration (denominators from the SCF
+// orbital energies throughout), generalised Fock from P + C G C^T, a damped
+// level-shifted rotation C <- C exp(-kappa).
+class CholeskyOMP2 {
+public:
+    CholeskyOMP2(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                 const SCFResult& scf_guess, const CholeskyOMP2Config& config = CholeskyOMP2Config());
+    CholeskyOMP2(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                 const SCFResult& scf_guess, const CholeskyOMP2Config& config,
+                 const integrals::CholeskyERI& cholesky_vectors);
+    OMP2Result compute();
+
+private:
+    Molecule mol_;
+    BasisSet basis_;
+    std::shared_ptr<IntegralEngine> integrals_;
+    SCFResult scf_;
+    CholeskyOMP2Config config_;
+    std::unique_ptr<integrals::CholeskyERI> internal_;
+    const integrals::CholeskyERI* chol_ = nullptr;
+    int nbf_, na_, nb_, va_, vb_;
+
+    void rotate(const Mat& w, const Mat& F_mo, int no, int nv, Mat& C, Mat& P) const;
 };
 
 }  // namespace mshqc

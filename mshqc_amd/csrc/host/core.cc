@@ -183,6 +183,39 @@ Vec solve(Mat A, Vec b) {
     return x;
 }
 
+Mat expm(const Mat& A) {
+    const int n = A.rows;
+    if (A.cols != n) throw std::invalid_argument("expm: matrix must be square");
+    double norm1 = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double c = 0.0;
+        for (int i = 0; i < n; ++i) c += std::fabs(A(i, j));
+        norm1 = std::max(norm1, c);
+    }
+    int sq = 0;
+    while (norm1 > 0.5) {
+        norm1 *= 0.5;
+        ++sq;
+    }
+    const Mat X = scale(A, std::ldexp(1.0, -sq));
+    // smallest m with ||X||^(m+1)/(m+1)! < 2^-60
+    int m = 1;
+    double term = norm1;
+    while (m < 30 && term * norm1 / (m + 1) >= std::ldexp(1.0, -60)) {
+        term *= norm1 / (m + 1);
+        ++m;
+    }
+    // Horner: E = I + X/1 (I + X/2 (I + ... (I + X/m)))
+    Mat E = Mat::identity(n);
+    for (int k = m; k >= 1; --k) {
+        Mat XE = scale(matmul(X, E), 1.0 / k);
+        for (int i = 0; i < n; ++i) XE(i, i) += 1.0;
+        E = XE;
+    }
+    for (int k = 0; k < sq; ++k) E = matmul(E, E);
+    return E;
+}
+
 // ================================================================ molecule
 
 const Atom& Molecule::atom(int i) const {

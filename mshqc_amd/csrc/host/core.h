@@ -51,6 +51,10 @@ void eigh(const Mat& A, Vec& w, Mat& V);
 // count as zero and their unknowns are set to 0, so a singular system (e.g.
 // vanishing DIIS error vectors) still returns a solution.
 Vec solve(Mat A, Vec b);
+// exp(A) by scaling and squaring with a Taylor polynomial: A is scaled by
+// 2^-s so that ||A||_1 <= 1/2, the series is summed (Horner) to the first
+// degree whose remainder bound is below 2^-60, then squared s times.
+Mat expm(const Mat& A);
 
 // ----------------------------------------------------------------- molecule
 

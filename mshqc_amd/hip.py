@@ -38,6 +38,8 @@ _SIGS = {
     "mshqc_hip_jk_factor": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _vp],
     "mshqc_hip_jk_rohf": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp],
     "mshqc_hip_half_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp],
+    "mshqc_hip_omp2_opdm": [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
+                            _D, _vp, _vp, _vp, _vp],
     "mshqc_hip_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp],
     "mshqc_hip_rmp2_energy": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D],
     "mshqc_hip_rmp2_grad": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D,
@@ -277,6 +279,21 @@ class Context:
             *[self._ptr(x) for x in d])
         self._check(rc, "mshqc_hip_ump2_grad")
         return tuple(E), (dBa, dBb, *d)
+
+    def omp2_opdm(self, Ba, Bb, eoa, eva, eob, evb):
+        """(E_aa, E_bb, E_ab), (Goo_a, Gvv_a, Goo_b, Gvv_b) of one OMP2 iteration."""
+        torch = self._torch
+        Q, oa, va = Ba.shape
+        _, ob, vb = Bb.shape
+        G = [torch.empty((k, k), dtype=torch.float64, device=Ba.device) for k in (oa, va, ob, vb)]
+        E = (ctypes.c_double * 3)()
+        self._follow_stream()
+        rc = lib().mshqc_hip_omp2_opdm(
+            self.h, self._ptr(Ba, name="Ba"), self._ptr(Bb, (Q, ob, vb), "Bb"), Q, oa, va, ob, vb,
+            self._ptr(eoa, (oa,)), self._ptr(eva, (va,)), self._ptr(eob, (ob,)), self._ptr(evb, (vb,)),
+            E, *[self._ptr(g) for g in G])
+        self._check(rc, "mshqc_hip_omp2_opdm")
+        return tuple(E), tuple(G)
 
 
 _CTX = {}

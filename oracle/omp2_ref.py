@@ -1,15 +1,22 @@
-"""numpy restatement of the CD-OMP2 amplitude, energy and one-particle
-density pieces -- TEST INFRASTRUCTURE ONLY (tiny sizes).
+"""numpy restatement of CD-OMP2 (src/mp2/cholesky_omp2.cc) -- TEST
+INFRASTRUCTURE ONLY (tiny sizes).
 
 * ``t2_blocks``  cholesky_omp2.cc:94-183  antisymmetrised t_aa, t_bb and t_ab
   from B^Q (skip |D| < 1e-12)
 * ``energy``     cholesky_omp2.cc:189-230  sum_{i<j,a<b} t^2 D (same spin) +
   sum t_ab^2 D (opposite spin)
-* ``opdm``       cholesky_omp2.cc:232-326  G_oo (negative) and G_vv per spin
-
-Its diagonals are the orbital-energy derivatives of the MP2 energy, which
-is how tests pin the backward kernel's dE/deps against the repository's own
-OMP2 orbital-gradient terms.
+* ``opdm``       cholesky_omp2.cc:232-326  G_oo (negative) and G_vv per spin.
+  Its diagonals are the orbital-energy derivatives of the MP2 energy, which
+  is how tests pin the backward kernel's dE/deps against the repository's
+  own OMP2 orbital-gradient terms.
+* ``cd_omp2``    cholesky_omp2.cc:331-513  the orbital loop: P + C G C^T per
+  spin, generalised Fock H + J - K_s in the MO basis, w = 2 F_vo, damped
+  level-shifted rotation C exp(-kappa) (scipy's expm); denominators keep
+  the SCF orbital energies; on exit the total is the previous iteration's.
+  Pinned by: iteration 1 reproducing the UMP2/RMP2 energies of the reference
+  build's own code (via the pinned pair oracle), a one-electron system
+  stopping at iteration 1 with the SCF energy (w = 2 F_vo is the SCF
+  orbital gradient), and orthonormal rotated orbitals (tests/test_omp2.py).
 """
 from __future__ import annotations
 
@@ -66,3 +73,78 @@ def opdm(taa, tbb, tab):
     Goo_b = -(0.5 * np.einsum("ikab,jkab->ij", tbb, tbb) + np.einsum("kiab,kjab->ij", tab, tab))
     Gvv_b = 0.5 * np.einsum("ijac,ijbc->ab", tbb, tbb) + np.einsum("ijca,ijcb->ab", tab, tab)
     return Goo_a, Gvv_a, Goo_b, Gvv_b
+
+
+def cd_omp2(H, L, C_a, C_b, P_a, P_b, eps_a, eps_b, na, nb, e_scf, max_iterations=50,
+            energy_threshold=1e-6, gradient_threshold=1e-5):
+    """The CD-OMP2 loop of cholesky_omp2.cc:427-513 with numpy and the C
+    oracle's transform and J/K.  Returns dict(energy_total, e_mp2, e_ss, e_os,
+    C_a, C_b, iterations, converged, trace) where trace lists (E_total,
+    E_mp2, g_norm) per iteration."""
+    import scipy.linalg as sl
+
+    from . import jk as _jk, transform as _transform
+
+    n = H.shape[0]
+    va, vb = n - na, n - nb
+    eoa, eva, eob, evb = eps_a[:na], eps_a[na:], eps_b[:nb], eps_b[nb:]
+    C_a, C_b, P_a, P_b = C_a.copy(), C_b.copy(), P_a.copy(), P_b.copy()
+    Q = L.shape[0]
+
+    def B_of(C, o):
+        if o == 0 or o == n:
+            return np.zeros((Q, o, n - o))
+        return _transform(L, C[:, :o], C[:, o:])
+
+    def rotate(w, F, o, C):
+        # :375-425: damped, level-shifted step, C <- C exp(-kappa)
+        kappa = np.zeros((n, n))
+        for a in range(n - o):
+            for i in range(o):
+                diff = F[o + a, o + a] - F[i, i] + 0.2
+                if abs(diff) < 1e-2:
+                    diff = 1e-2 if diff >= 0 else -1e-2
+                val = (w[a, i] / diff) * 0.5
+                kappa[o + a, i] = val
+                kappa[i, o + a] = -val
+        C = C @ sl.expm(-kappa)
+        return C, C[:, :o] @ C[:, :o].T
+
+    e_prev = e_scf
+    e_mp2 = e_ss = e_os = 0.0
+    converged = False
+    trace = []
+    it = 0
+    while it < max_iterations:
+        it += 1
+        Ba, Bb = B_of(C_a, na), B_of(C_b, nb)
+        taa, tbb, tab = t2_blocks(Ba, Bb, eoa, eva, eob, evb)
+        e_mp2 = energy(taa, tbb, tab, eoa, eva, eob, evb)
+        e_os = float(np.sum(tab * tab * _denom(eoa, eva, eob, evb)))
+        e_ss = e_mp2 - e_os
+        e_curr = e_scf + e_mp2
+        Goo_a, Gvv_a, Goo_b, Gvv_b = opdm(taa, tbb, tab)
+        Pt = []
+        for C, P, Goo, Gvv, o in ((C_a, P_a, Goo_a, Gvv_a, na), (C_b, P_b, Goo_b, Gvv_b, nb)):
+            G = np.zeros((n, n))
+            G[:o, :o] = Goo
+            G[o:, o:] = Gvv
+            Pt.append(P + C @ G @ C.T)
+        J, K = _jk(L, np.stack(Pt))
+        F_a = C_a.T @ (H + J - K[0]) @ C_a
+        F_b = C_b.T @ (H + J - K[1]) @ C_b
+        w_a = 2.0 * F_a[na:, :na]
+        w_b = 2.0 * F_b[nb:, :nb]
+        g_norm = float(np.sqrt(np.sum(w_a * w_a) + np.sum(w_b * w_b)))
+        trace.append((e_curr, e_mp2, g_norm))
+        if g_norm < gradient_threshold and abs(e_curr - e_prev) < energy_threshold:
+            converged = True
+            break
+        C_a, P_a = rotate(w_a, F_a, na, C_a)
+        if na != nb:
+            C_b, P_b = rotate(w_b, F_b, nb, C_b)
+        else:
+            C_b, P_b = C_a, P_a
+        e_prev = e_curr
+    return dict(energy_total=e_prev, e_mp2=e_mp2, e_ss=e_ss, e_os=e_os, C_a=C_a, C_b=C_b,
+                iterations=it, converged=converged, trace=trace)

@@ -1,0 +1,111 @@
+"""CD-OMP2: the oracle's restatement of the reference loop
+(src/mp2/cholesky_omp2.cc:331-513) pinned by what the mathematics fixes, and
+the host matrix exponential used by the orbital rotation.  Runs on the CPU
+with the build's own integral engine and host Cholesky decomposition."""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+import oracle
+from conftest import ROOT
+from oracle import omp2_ref, scf_ref
+
+BD = os.path.join(ROOT, "mshqc_amd", "data", "basis")
+
+
+@pytest.fixture(scope="module")
+def m():
+    from mshqc_amd import native_module
+
+    return native_module()
+
+
+def system(m, atoms, basis, thr=1e-8, mult=1):
+    mol = m.Molecule(0, mult)
+    for Z, x, y, z in atoms:
+        mol.add_atom(Z, x, y, z)
+    b = m.BasisSet(basis, mol, BD)
+    eng = m.IntegralEngine(mol, b)
+    chol = m.CholeskyERI(thr)
+    chol.decompose(eng.compute_eri())
+    n = b.n_basis_functions()
+    L = np.array(chol.get_L_vectors()).reshape(-1, n, n)
+    S = eng.compute_overlap()
+    H = eng.compute_kinetic() + eng.compute_nuclear()
+    return mol, S, H, L
+
+
+def rhf_guess(mol, S, H, L, nocc):
+    r = scf_ref.cd_rhf(S, H, L, nocc, mol.nuclear_repulsion_energy(), energy_threshold=1e-11, max_iterations=300)
+    return dict(C_a=r["C"], C_b=r["C"], P_a=0.5 * r["P"], P_b=0.5 * r["P"], eps_a=r["eps"], eps_b=r["eps"],
+                e_scf=r["energy"])
+
+
+def uhf_guess(mol, S, H, L, na, nb):
+    r = scf_ref.cd_uhf(S, H, L, na, nb, mol.nuclear_repulsion_energy(), energy_threshold=1e-11,
+                       max_iterations=300)
+    return dict(C_a=r["Ca"], C_b=r["Cb"], P_a=r["Pa"], P_b=r["Pb"], eps_a=r["ea"], eps_b=r["eb"],
+                e_scf=r["energy"])
+
+
+def test_first_iteration_is_rmp2(m):
+    """RHF orbitals: the first iteration's amplitude energy is the CD-RMP2
+    energy, whose oracle matches the reference's compiled CholeskyRMP2."""
+    mol, S, H, L = system(m, [(1, 0.0, 0.0, 1.4 * k) for k in range(4)], "cc-pvdz")
+    g = rhf_guess(mol, S, H, L, 2)
+    r = omp2_ref.cd_omp2(H, L, g["C_a"], g["C_b"], g["P_a"], g["P_b"], g["eps_a"], g["eps_b"], 2, 2, g["e_scf"],
+                         max_iterations=3)
+    C, eps = g["C_a"], g["eps_a"]
+    e_rmp2 = oracle.rmp2_energy(oracle.transform(L, C[:, :2], C[:, 2:]), eps[:2], eps[2:])
+    assert abs(r["trace"][0][1] - e_rmp2) < 1e-12 * abs(e_rmp2)
+    assert abs(r["trace"][0][0] - (g["e_scf"] + e_rmp2)) < 1e-12
+    # the loop lowers the total below the MP2 one
+    assert r["trace"][-1][0] < r["trace"][0][0]
+    # converged: the total on exit is the one before the last iteration
+    r = omp2_ref.cd_omp2(H, L, g["C_a"], g["C_b"], g["P_a"], g["P_b"], g["eps_a"], g["eps_b"], 2, 2, g["e_scf"])
+    assert r["converged"] and r["energy_total"] == r["trace"][-2][0]
+
+
+def test_first_iteration_is_ump2_and_orbitals_stay_orthonormal(m):
+    """UHF orbitals (H3 chain doublet, 2 alpha / 1 beta, cc-pVDZ): iteration 1
+    gives the CD-UMP2 components; rotated orbitals stay S-orthonormal."""
+    mol, S, H, L = system(m, [(1, 0.0, 0.0, 1.4 * k) for k in range(3)], "cc-pvdz", mult=2)
+    g = uhf_guess(mol, S, H, L, 2, 1)
+    r = omp2_ref.cd_omp2(H, L, g["C_a"], g["C_b"], g["P_a"], g["P_b"], g["eps_a"], g["eps_b"], 2, 1, g["e_scf"],
+                         max_iterations=6, energy_threshold=1e-12, gradient_threshold=1e-10)
+    Ca, Cb, ea, eb = g["C_a"], g["C_b"], g["eps_a"], g["eps_b"]
+    E = oracle.ump2(oracle.transform(L, Ca[:, :2], Ca[:, 2:]), oracle.transform(L, Cb[:, :1], Cb[:, 1:]),
+                    ea[:2], ea[2:], eb[:1], eb[1:])
+    assert abs(r["trace"][0][1] - sum(E)) < 1e-12 * abs(sum(E))
+    assert r["iterations"] == 6 and not r["converged"]
+    for C in (r["C_a"], r["C_b"]):
+        assert np.abs(C.T @ S @ C - np.eye(C.shape[1])).max() < 1e-12
+    assert not np.allclose(r["C_a"], Ca)  # the orbitals did move
+    assert r["energy_total"] == r["trace"][-1][0]  # not converged: the last iteration's total
+
+
+def test_one_electron_stops_at_scf(m):
+    """H atom (1 alpha, 0 beta): no pairs, so G = 0 and w = 2 F_vo is the
+    converged SCF orbital gradient (zero): the loop stops at iteration 1 with
+    the SCF energy."""
+    mol, S, H, L = system(m, [(1, 0.0, 0.0, 0.0)], "cc-pvdz", mult=2)
+    g = uhf_guess(mol, S, H, L, 1, 0)
+    r = omp2_ref.cd_omp2(H, L, g["C_a"], g["C_b"], g["P_a"], g["P_b"], g["eps_a"], g["eps_b"], 1, 0, g["e_scf"])
+    assert r["converged"] and r["iterations"] == 1
+    assert r["e_mp2"] == 0.0 and r["energy_total"] == g["e_scf"]
+    assert r["trace"][0][2] < 1e-9
+
+
+@pytest.mark.parametrize("scale", [1e-3, 0.3, 4.0])
+def test_host_expm(m, scale):
+    rng = np.random.default_rng(int(scale * 1000))
+    A = rng.standard_normal((17, 17))
+    K = scale * (A - A.T)
+    E = m.linalg_expm(K)
+    E_ref = sl.expm(K)
+    assert np.abs(E - E_ref).max() < 1e-13 * max(1.0, np.abs(E_ref).max())
+    assert np.abs(E.T @ E - np.eye(17)).max() < 1e-13  # exp of antisymmetric is orthogonal
+    G = scale * rng.standard_normal((9, 9))
+    assert np.abs(m.linalg_expm(G) - sl.expm(G)).max() < 1e-12 * max(1.0, np.abs(sl.expm(G)).max())

@@ -109,6 +109,9 @@ int ctx_finish(mshqc_hip_ctx* ctx);
 // reduce_batch = 1: sum over b into one output; the batch is split over
 //   `splits` workgroup layers whose partials land in `part`
 //   ([splits][M][N]) and are then summed in split order (deterministic).
+// inner > 1 (reduce_batch only): batch entry b = bo * inner + bi starts at
+//   A + bo*sA + bi*sA2 and X + bo*sX + bi*sX2, so one contraction can be cut
+//   into chunks along a second stride.
 struct GemmF64 {
     const double* A = nullptr;
     long lda = 0, sA = 0;
@@ -123,6 +126,8 @@ struct GemmF64 {
     bool accumulate = false;  // C += ... instead of C = ...
     bool reduce_batch = false;
     int splits = 1;
+    int inner = 1;
+    long sA2 = 0, sX2 = 0;
 };
 int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g);
 

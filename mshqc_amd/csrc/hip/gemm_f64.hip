@@ -79,8 +79,9 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmF64 g, double* __restr
     auto fetch = [&](long step) {
         const int b = b_lo + (int)(step / nk);
         const int k0 = (int)(step % nk) * KC;
-        const double* Ab = g.A + (long)b * g.sA;
-        const double* Xb = g.X + (long)b * g.sX;
+        const int bo = b / g.inner, bi = b - bo * g.inner;
+        const double* Ab = g.A + (long)bo * g.sA + (long)bi * g.sA2;
+        const double* Xb = g.X + (long)bo * g.sX + (long)bi * g.sX2;
         Panel<!TA>::load(Ab, g.lda, g.M, g.K, m0, k0, tid, ra);
         Panel<TX>::load(Xb, g.ldx, g.N, g.K, n0, k0, tid, rx);
     };
@@ -170,6 +171,8 @@ __global__ __launch_bounds__(1024) void reduce_sum_kernel(const double* __restri
 int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return MSHQC_HIP_OK;
     if (g.K <= 0) return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: K must be positive");
+    if (g.inner < 1 || (g.inner > 1 && !g.reduce_batch))
+        return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: two-level batches need reduce_batch");
     const unsigned gx = grouped_grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN);
     int splits = 1, bper = 1;
     double* part = nullptr;

@@ -75,12 +75,12 @@ struct Want {
     double* e_slot = nullptr;
 };
 
-// Batch layers of a batch-reduced GEMM: at most 256, and partials
+// Batch layers of a batch-reduced GEMM: at most 1024, and partials
 // ([splits][M][N]) within 2^25 doubles.
 int red_splits(int batch, long M, long N) {
     const long cap = 1L << 25;
     const long s = std::min<long>(batch, std::max<long>(1, cap / std::max<long>(M * N, 1)));
-    return (int)std::min<long>(s, 256);
+    return (int)std::min<long>(s, 1024);
 }
 
 // Largest k block whose V and T blocks (oX vX vY doubles per k) fit `cap`.
@@ -122,22 +122,25 @@ int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int k
         LAUNCH_CHECK(ctx);
         if (w.e_slot && (rc = reduce_sum(ctx, part, groups, w.e_slot, true))) return rc;
         const long vv = (long)s.vX * s.vY;
-        if (w.GooX) {  // sum_k T_k T_k^T, T_k = [oX][vX vY]
+        if (w.GooX) {  // sum_k T_k T_k^T, T_k = [oX][vX vY], cut into one chunk per a
             GemmF64 g;
             g.A = T;
             g.lda = vv;
             g.sA = per_k;
+            g.sA2 = s.vY;
             g.X = T;
             g.ldx = vv;
             g.sX = per_k;
+            g.sX2 = s.vY;
+            g.inner = s.vX;
             g.transX = true;
             g.C = w.GooX;
             g.ldc = s.oX;
             g.M = g.N = s.oX;
-            g.K = (int)vv;
-            g.batch = nk;
+            g.K = s.vY;
+            g.batch = nk * s.vX;
             g.reduce_batch = true;
-            g.splits = red_splits(nk, s.oX, s.oX);
+            g.splits = red_splits(g.batch, s.oX, s.oX);
             g.alpha = so;
             g.accumulate = true;
             { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
@@ -185,22 +188,25 @@ int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int k
             { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
             if (rc) return rc;
         }
-        if (w.GooY) {  // one block holds every k: -sum_i T_(.i) T_(.i)^T over (a, b)
+        if (w.GooY) {  // one block holds every k: -sum_{i,a} T_(.ia) T_(.ia)^T over b
             GemmF64 g;
             g.A = T;
             g.lda = per_k;
             g.sA = vv;
+            g.sA2 = s.vY;
             g.X = T;
             g.ldx = per_k;
             g.sX = vv;
+            g.sX2 = s.vY;
+            g.inner = s.vX;
             g.transX = true;
             g.C = w.GooY;
             g.ldc = s.oY;
             g.M = g.N = nk;
-            g.K = (int)vv;
-            g.batch = s.oX;
+            g.K = s.vY;
+            g.batch = s.oX * s.vX;
             g.reduce_batch = true;
-            g.splits = red_splits(s.oX, nk, nk);
+            g.splits = red_splits(g.batch, nk, nk);
             g.alpha = -1.0;
             g.accumulate = true;
             { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }

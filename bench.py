@@ -337,6 +337,34 @@ def run_omp2(args, torch, hip, syn, w):
     return line
 
 
+def run_cd(args, torch, hip, syn, w):
+    """Pivoted Cholesky of an ERI-shaped PSD matrix M = L^T L (n^2 x n^2) on
+    the device (cholesky_eri.cc:40-203)."""
+    ctx = hip.context(0)
+    Q0, n, thr = w["Q"], w["n"], w["thr"]
+    np2 = n * n
+    log(f"[bench] generating {args.workload}: M = L^T L, L [{Q0}, {np2}]")
+    Ls = torch.from_numpy(syn.workload(args.workload)).cuda()
+    M = (Ls.t() @ Ls).contiguous()
+    del Ls
+    sec, out, _ = timed(torch, ctx, lambda: ctx.cholesky(M, thr), args.steps, args.warmup)
+    L, conv, res = out
+    q = L.shape[0]
+    # HBM bytes the algorithm moves: step k reads k earlier vectors and the
+    # pivot column, writes one vector, scans and updates the diagonal
+    traffic = sum(8.0 * np2 * (k + 1 + 1 + 3) for k in range(q))
+    achieved = traffic / sec / 1e9
+    log(f"[bench] cholesky: {q} vectors (converged={conv}, max residual {res:.2e}) in {sec * 1e3:.1f} ms, "
+        f"{achieved:.0f} GB/s algorithmic")
+    config = {"workload": f"{args.workload}: pivoted Cholesky of an n^2 x n^2 PSD matrix", "n": n,
+              "rank": Q0, "threshold": thr, "vectors": q, "parallelism": "single GPU"}
+    line = base_line(args, sec, "f64", config, 0.0)
+    line["roofline"] = {"kernel": "pivot scan + new vector (reads the earlier vectors)", "bound": "hbm",
+                        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -367,6 +395,8 @@ def main():
         line = run_mp2(args, torch, hip, syn, w)
     elif w["kind"] == "omp2":
         line = run_omp2(args, torch, hip, syn, w)
+    elif w["kind"] == "cd":
+        line = run_cd(args, torch, hip, syn, w)
     else:
         line = run_jk(args, torch, hip, syn, w)
     print(json.dumps(line), flush=True)

@@ -82,6 +82,28 @@ int mshqc_hip_free(mshqc_hip_ctx* ctx, void* dptr);
 int mshqc_hip_memcpy_h2d(mshqc_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
 int mshqc_hip_memcpy_d2h(mshqc_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
 
+/* ----------------------------------------------------- Cholesky vectors */
+
+/*
+ * Pivoted incomplete Cholesky of the ERI matrix, on the device in FP64.
+ * Replaces CholeskyERI::decompose (src/integrals/cholesky_eri.cc:40-203):
+ *   diagonal D(ij) = (ij|ij); repeatedly take the first maximum p of D, stop
+ *   when D_p < threshold or max_vectors vectors exist;
+ *   L_new(ij) = ((ij|p) - sum_k L_k(ij) L_k(p)) / sqrt(D_p),
+ *   D(ij) -= L_new(ij)^2, negative values clamped to 0.
+ * eri: DEVICE [n*n][n*n], element (ij|kl) at (i*n+j)*n*n + k*n+l.
+ * L:   DEVICE [max_vectors][n*n], vectors in pivot order.
+ * Operation order and rounding follow the reference's scalar loop (no fused
+ * multiply-adds), so vectors and pivots equal the FP64 host algorithm's.
+ * Host outputs: *n_vectors; *converged = 1 when the threshold was reached
+ * (0 when max_vectors stopped it, as the reference's n*n bound does);
+ * *max_residual = the largest remaining diagonal.  converged and
+ * max_residual may be NULL.
+ */
+int mshqc_hip_cholesky(mshqc_hip_ctx* ctx, const double* eri, int n, double threshold,
+                       int max_vectors, double* L, int* n_vectors, int* converged,
+                       double* max_residual);
+
 /* --------------------------------------------------------- Fock pieces */
 
 /*

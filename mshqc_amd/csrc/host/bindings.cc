@@ -186,6 +186,7 @@ PYBIND11_MODULE(_mshqc, m) {
     py::class_<CholeskyERI>(m, "CholeskyERI")
         .def(py::init<double>(), py::arg("threshold") = 1e-6)
         .def("decompose", &CholeskyERI::decompose)
+        .def("decompose_gpu", &CholeskyERI::decompose_gpu)
         .def("get_L_vectors",
              [](const CholeskyERI& c) {
                  py::list out;

@@ -176,6 +176,42 @@ CholeskyDecompositionResult CholeskyERI::decompose(const ERITensor& eri) {
     return res;
 }
 
+CholeskyDecompositionResult CholeskyERI::decompose_gpu(const ERITensor& eri) {
+    // cholesky_eri.cc:40-203 on the device
+    clear();
+    n_basis_ = eri.n;
+    const int n = eri.n;
+    const long np = (long)n * n;
+    CholeskyDecompositionResult res;
+    res.n_basis = n;
+    res.threshold = threshold_;
+    if (np == 0) {
+        decomposed_ = true;
+        res.converged = true;
+        return res;
+    }
+    // the reference's bound is n^2 vectors; hold that many when the buffer is
+    // at most the size of the ERI matrix itself, which is always the case
+    const int cap = (int)np;
+    auto dM = std::make_unique<device::Buffer>((size_t)np * np);
+    dM->upload(eri.v.data(), (size_t)np * np);
+    device::Buffer dL((size_t)cap * np);
+    int nv = 0, conv = 0;
+    double resid = 0.0;
+    device::check(mshqc_hip_cholesky(device::ctx(), dM->get(), n, threshold_, cap, dL.get(), &nv, &conv, &resid),
+                  "mshqc_hip_cholesky");
+    dM.reset();
+    Vec host((size_t)nv * np);
+    if (nv > 0) dL.download(host.data(), host.size());
+    L_.assign((size_t)nv, Vec());
+    for (int q = 0; q < nv; ++q) L_[q].assign(host.begin() + (size_t)q * np, host.begin() + (size_t)(q + 1) * np);
+    decomposed_ = true;
+    res.n_vectors = nv;
+    res.compression_ratio = compression_ratio();
+    res.converged = conv != 0;
+    return res;
+}
+
 double CholeskyERI::reconstruct(int i, int j, int a, int b) const {
     if (!decomposed_) throw std::runtime_error("CholeskyERI::reconstruct: not decomposed");
     const int ij = i * n_basis_ + j, ab = a * n_basis_ + b;
@@ -212,7 +248,7 @@ CholeskyRHF::CholeskyRHF(const Molecule& mol, const BasisSet& basis,
         throw std::runtime_error("CholeskyRHF: Molecule must be closed-shell (even electrons).");
     n_occ_ = ne / 2;
     internal_ = std::make_unique<integrals::CholeskyERI>(config_.cholesky_threshold);
-    internal_->decompose(integrals_->compute_eri());
+    internal_->decompose_gpu(integrals_->compute_eri());
     chol_ = internal_.get();
 }
 
@@ -348,7 +384,7 @@ void CholeskyUHF::init_integrals_and_cholesky() {
     S_ = integrals_->compute_overlap();
     H_ = add(integrals_->compute_kinetic(), integrals_->compute_nuclear());
     X_ = inverse_sqrt(S_);
-    if (!external_ && chol_.n_vectors() == 0) chol_.decompose(integrals_->compute_eri());
+    if (!external_ && chol_.n_vectors() == 0) chol_.decompose_gpu(integrals_->compute_eri());
 }
 
 void CholeskyUHF::solve_fock(const Mat& F, Mat& C, Vec& eps) const {
@@ -494,7 +530,7 @@ void CholeskyROHF::init_integrals() {
     H_ = add(integrals_->compute_kinetic(), integrals_->compute_nuclear());
     if (chol_.n_vectors() == 0) {
         if (config_.print_level > 0) std::printf("[CholeskyROHF] Decomposing Integrals...\n");
-        chol_.decompose(integrals_->compute_eri());
+        chol_.decompose_gpu(integrals_->compute_eri());
     }
     if (chol_.n_basis() != nbasis_)
         throw std::runtime_error("CholeskyROHF: Cholesky vectors do not match the basis");
@@ -630,7 +666,7 @@ CholeskyRMP2::CholeskyRMP2(const Molecule&, const BasisSet& basis, std::shared_p
     nocc_ = rhf_.n_occ_alpha;
     nvirt_ = nbf_ - nocc_;
     internal_ = std::make_unique<integrals::CholeskyERI>(config_.cholesky_threshold);
-    internal_->decompose(integrals->compute_eri());
+    internal_->decompose_gpu(integrals->compute_eri());
     chol_ = internal_.get();
 }
 
@@ -702,7 +738,7 @@ CholeskyUMP2Result CholeskyUMP2::compute() {
     CholeskyUMP2Result r;
     auto t0 = std::chrono::high_resolution_clock::now();
     if (chol_.n_vectors() == 0) {
-        chol_.decompose(integrals_->compute_eri());
+        chol_.decompose_gpu(integrals_->compute_eri());
         r.time_cholesky_s = seconds_since(t0);
     }
     const int Q = chol_.n_vectors();
@@ -749,7 +785,7 @@ CholeskyOMP2::CholeskyOMP2(const Molecule& mol, const BasisSet& basis, std::shar
     if (config_.print_level > 0)
         std::printf("[CholeskyOMP2] Decomposing integrals (Thresh=%g)...\n", config_.cholesky_threshold);
     internal_ = std::make_unique<integrals::CholeskyERI>(config_.cholesky_threshold);
-    internal_->decompose(integrals_->compute_eri());
+    internal_->decompose_gpu(integrals_->compute_eri());
     chol_ = internal_.get();
 }
 

@@ -62,6 +62,10 @@ class CholeskyERI {
 public:
     explicit CholeskyERI(double threshold = 1e-6);
     CholeskyDecompositionResult decompose(const ERITensor& eri);
+    // The same decomposition on the device (mshqc_hip_cholesky): identical
+    // vectors and pivots; the ERI matrix is uploaded once and the vectors come
+    // back to the host.
+    CholeskyDecompositionResult decompose_gpu(const ERITensor& eri);
     double reconstruct(int i, int j, int a, int b) const;
     int n_vectors() const { return (int)L_.size(); }
     int n_basis() const { return n_basis_; }

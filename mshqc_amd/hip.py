@@ -38,6 +38,8 @@ _SIGS = {
     "mshqc_hip_jk_factor": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _vp],
     "mshqc_hip_jk_rohf": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp],
     "mshqc_hip_half_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp],
+    "mshqc_hip_cholesky": [_vp, _vp, _c_int, ctypes.c_double, _c_int, _vp, ctypes.POINTER(_c_int),
+                           ctypes.POINTER(_c_int), _D],
     "mshqc_hip_omp2_opdm": [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                             _D, _vp, _vp, _vp, _vp],
     "mshqc_hip_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp],
@@ -176,6 +178,23 @@ class Context:
                                        self._ptr(Y, (n, m), "Y"), m, self._ptr(J), self._ptr(K))
         self._check(rc, "mshqc_hip_jk_factor")
         return J, K
+
+    def cholesky(self, eri, threshold, max_vectors=None):
+        """Pivoted Cholesky of a device ERI matrix [n*n, n*n] (or [n,n,n,n]).
+        Returns (L [Q, n*n] device tensor, converged, max_residual)."""
+        torch = self._torch
+        np2 = eri.shape[0] if eri.dim() == 2 else eri.shape[0] * eri.shape[1]
+        n = int(round(np2 ** 0.5))
+        if n * n != np2 or eri.numel() != np2 * np2:
+            raise ValueError("eri: need [n*n, n*n] or [n, n, n, n]")
+        cap = np2 if max_vectors is None else int(max_vectors)
+        L = torch.empty((cap, np2), dtype=torch.float64, device=eri.device)
+        nv, conv, res = _c_int(), _c_int(), ctypes.c_double()
+        self._follow_stream()
+        rc = lib().mshqc_hip_cholesky(self.h, self._ptr(eri, name="eri"), n, float(threshold), cap,
+                                      self._ptr(L), ctypes.byref(nv), ctypes.byref(conv), ctypes.byref(res))
+        self._check(rc, "mshqc_hip_cholesky")
+        return L[: nv.value], bool(conv.value), res.value
 
     def jk_rohf(self, L, C, na, nb):
         """J[Pa + Pb], Ka, Kb for alpha = C[:, :na], beta = C[:, :nb]."""

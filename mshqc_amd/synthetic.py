@@ -17,6 +17,7 @@ Named workloads (sizes):
 ``mp2_o40``     B: Q=1600, o=40, v=360                 (differentiation form)
 ``ump2_o120``   Q=5600, o_a=120, v_a=1380, o_b=118, v_b=1382 (CD-UMP2)
 ``omp2_o40``    Q=1600, o_a=40, v_a=360, o_b=38, v_b=362  (CD-OMP2 amplitudes + OPDM)
+``cd_n100``     ERI matrix M = L^T L from L [1500, 100*100] (pivoted Cholesky, thr 1e-6)
 ==============  ==========================================================
 """
 from __future__ import annotations
@@ -30,6 +31,7 @@ WORKLOADS = {
     "mp2_o40": dict(kind="rmp2", Q=1600, o=40, v=360, seed=1004),
     "ump2_o120": dict(kind="ump2", Q=5600, oa=120, va=1380, ob=118, vb=1382, seed=1005),
     "omp2_o40": dict(kind="omp2", Q=1600, oa=40, va=360, ob=38, vb=362, seed=1006),
+    "cd_n100": dict(kind="cd", Q=1500, n=100, thr=1e-6, seed=1007),
 }
 
 
@@ -94,6 +96,9 @@ def workload(name):
     w = WORKLOADS[name]
     if w["kind"] == "rmp2":
         return rmp2_inputs(w["Q"], w["o"], w["v"], w["seed"])
+    if w["kind"] == "cd":
+        rng = np.random.default_rng(w["seed"])
+        return gen_L(w["Q"], w["n"], rng).reshape(w["Q"], w["n"] * w["n"])
     if w["kind"] in ("ump2", "omp2"):
         return ump2_inputs(w["Q"], w["oa"], w["va"], w["ob"], w["vb"], w["seed"])
     return jk_inputs(w["Q"], w["n"], w["o"], w["seed"])

@@ -365,6 +365,44 @@ def run_cd(args, torch, hip, syn, w):
     return line
 
 
+def rmp3_flops(Q, o, v):
+    """FP64 flops of the CD-RMP3 call: (ac|bd) from Lvv (2 v^4 Q), the
+    particle-particle contraction (2 o^2 v^4), (ki|lj) and the hole-hole term
+    (2 o^4 Q + 2 o^4 v^2), (ia|jb) (2 (ov)^2 Q) and the ring product 2 (ov)^3."""
+    return 2.0 * v ** 4 * Q + 2.0 * o * o * v ** 4 + 2.0 * o ** 4 * (Q + v * v) + 2.0 * (o * v) ** 2 * Q \
+        + 2.0 * (o * v) ** 3
+
+
+def run_rmp3(args, torch, hip, syn, w):
+    """CD-RMP3 second-order amplitudes and energy (cholesky_rmp3.cc:79-200)
+    from device MO blocks; the transforms and t1 are made before timing."""
+    ctx = hip.context(0)
+    Q, n, o = w["Q"], w["n"], w["o"]
+    v = n - o
+    log(f"[bench] generating {args.workload}: Q={Q} n={n} o={o} v={v}")
+    L, C, eo, ev = syn.workload(args.workload)
+    dL = torch.from_numpy(L).cuda()
+    del L
+    Co = torch.from_numpy(np.ascontiguousarray(C[:, :o])).cuda()
+    Cv = torch.from_numpy(np.ascontiguousarray(C[:, o:])).cuda()
+    deo, dev = torch.from_numpy(eo).cuda(), torch.from_numpy(ev).cuda()
+    B, Loo, Lvv = ctx.transform(dL, Co, Cv), ctx.transform(dL, Co, Co), ctx.transform(dL, Cv, Cv)
+    t1 = ctx.rmp2_t2(B, deo, dev)
+    sec, out, prof = timed(torch, ctx, lambda: ctx.rmp3(B, Loo, Lvv, t1, deo, dev), args.steps, args.warmup)
+    flop = rmp3_flops(Q, o, v)
+    log(f"[bench] rmp3: {sec * 1e3:.1f} ms/step, {flop / sec / 1e12:.1f} TFLOP/s FP64, E3 = {out[0]!r}, "
+        f"GEMMs {prof[1]}")
+    config = {"workload": f"{args.workload}: CD-RMP3 second-order amplitudes and energy", "Q": Q, "n": n,
+              "o": o, "v": v, "parallelism": "single GPU"}
+    line = base_line(args, sec, "f64", config, flop)
+    achieved = flop / sec / 1e12
+    line["roofline"] = {"kernel": "FP64 MFMA GEMMs ((ac|bd) build and ladder contractions)", "bound": "mfma",
+                        "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s", "frac": achieved / 78.6,
+                        "traffic": None}
+    line["e_mp3"] = out[0]
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -397,6 +435,8 @@ def main():
         line = run_omp2(args, torch, hip, syn, w)
     elif w["kind"] == "cd":
         line = run_cd(args, torch, hip, syn, w)
+    elif w["kind"] == "rmp3":
+        line = run_rmp3(args, torch, hip, syn, w)
     else:
         line = run_jk(args, torch, hip, syn, w)
     print(json.dumps(line), flush=True)

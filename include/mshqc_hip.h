@@ -242,6 +242,26 @@ int mshqc_hip_omp2_opdm(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
                         const double* eps_ob, const double* eps_vb, double* E,
                         double* Goo_a, double* Gvv_a, double* Goo_b, double* Gvv_b);
 
+/* ---------------------------------------------------------------- RMP3 */
+
+/*
+ * CD-RMP3 second-order amplitudes and energy, replacing the t2(2) and E(3)
+ * loops of foundation::CholeskyRMP3::compute (src/mp3/cholesky_rmp3.cc:
+ * 100-178).  Inputs: B [Q][o][v], Loo [Q][o][o], Lvv [Q][v][v] (each
+ * C_left^T L^Q C_right, e.g. from mshqc_hip_transform), the first-order
+ * amplitudes t1 [o][o][v][v] (the CD-RMP2 t2) and the orbital energies.
+ *   D t2_2(ijab) = 1/2 sum_cd [(ac|bd) - (ad|bc)] t1(ijcd)
+ *                + 1/2 sum_kl [(ki|lj) - (kj|li)] t1(klab)
+ *                - sum_kc [2(jb|kc) - (kb|jc)] t1(ikac)
+ *                - sum_kc [2(ia|kc) - (ka|ic)] t1(jkbc)
+ * (|D| < 1e-12 gives 0);  *E3 (HOST) = sum_ijab [2(ia|jb) - (ib|ja)] t2_2.
+ * t2_2 (DEVICE [o][o][v][v]) may be NULL.  Workspace: about 5 o^2 v^2 doubles
+ * plus (ac|bd) blocks of at most twice the mshqc_hip_set_zblock cap.
+ */
+int mshqc_hip_rmp3(mshqc_hip_ctx* ctx, const double* B, const double* Loo,
+                   const double* Lvv, int Q, int o, int v, const double* t1,
+                   const double* eps_o, const double* eps_v, double* E3, double* t2_2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,7 @@ struct mshqc_hip_ctx {
         void* ptr = nullptr;
         size_t bytes = 0;
     };
-    Slab slabs[12];
+    Slab slabs[16];
     // Event pair for timing the last compute call (milliseconds).
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
@@ -65,6 +65,10 @@ enum WsSlot {
     WS_ZTS = 9,    // ... and of its transpose
     WS_OMP2_V = 10,  // OMP2 pair-integral block (ia|kb)
     WS_OMP2_T = 11,  // OMP2 amplitude block
+    WS_MP3_A = 12,   // RMP3 operands and intermediates
+    WS_MP3_B = 13,
+    WS_MP3_C = 14,
+    WS_MP3_D = 15,
 };
 
 // Sets ctx->last_error and returns the status code.

@@ -358,6 +358,28 @@ PYBIND11_MODULE(_mshqc, m) {
             return std::move(a);
         });
 
+    // bindings.cc:395-399 (RMP3Result) and 594-600 (CholeskyRMP3) of the reference
+    py::class_<foundation::RMP3Result>(m, "RMP3Result")
+        .def(py::init<>())
+        .def_readwrite("e_rhf", &foundation::RMP3Result::e_rhf)
+        .def_readwrite("e_mp2", &foundation::RMP3Result::e_mp2)
+        .def_readwrite("e_mp3", &foundation::RMP3Result::e_mp3)
+        .def_readwrite("e_corr_total", &foundation::RMP3Result::e_corr_total)
+        .def_readwrite("e_total", &foundation::RMP3Result::e_total)
+        .def_readonly("n_occ", &foundation::RMP3Result::n_occ)
+        .def_readonly("n_virt", &foundation::RMP3Result::n_virt)
+        .def_property_readonly("t2_2", [](const foundation::RMP3Result& r) -> py::object {
+            if (r.t2_2.empty()) return py::none();
+            py::array_t<double> a({r.n_occ, r.n_occ, r.n_virt, r.n_virt});
+            std::copy(r.t2_2.begin(), r.t2_2.end(), a.mutable_data());
+            return std::move(a);
+        });
+
+    py::class_<foundation::CholeskyRMP3>(m, "CholeskyRMP3")
+        .def(py::init<const SCFResult&, const foundation::CholeskyRMP2Result&, const BasisSet&>(),
+             py::arg("rhf_result"), py::arg("crmp2_result"), py::arg("basis"))
+        .def("compute", &foundation::CholeskyRMP3::compute);
+
     py::class_<CholeskyRMP2>(m, "CholeskyRMP2")
         .def(py::init<const Molecule&, const BasisSet&, std::shared_ptr<IntegralEngine>, const SCFResult&,
                       const CholeskyRMP2Config&>(),

@@ -220,6 +220,11 @@ double CholeskyERI::reconstruct(int i, int j, int a, int b) const {
     return s;
 }
 
+std::shared_ptr<device::Buffer> CholeskyERI::device_buffer() const {
+    device_vectors();
+    return dev_;
+}
+
 const double* CholeskyERI::device_vectors() const {
     if (L_.empty()) throw std::runtime_error("CholeskyERI: no vectors (decompose or set them first)");
     if (!dev_) {
@@ -712,11 +717,83 @@ foundation::CholeskyRMP2Result CholeskyRMP2::compute() {
     r.n_occ = o;
     r.n_virt = v;
     r.n_chol_vectors = Q;
+    r.chol_device = chol_->device_buffer();
+    r.n_basis = nbf_;
     if (config_.print_level > 0)
         std::printf("  RHF Energy:    %16.10f Ha\n  Cholesky RMP2: %16.10f Ha\n  Total Energy:  %16.10f Ha\n",
                     r.e_rhf, r.e_corr, r.e_total);
     return r;
 }
+
+// ================================================================ CD-RMP3
+
+namespace foundation {
+
+CholeskyRMP3::CholeskyRMP3(const SCFResult& rhf_result, const CholeskyRMP2Result& crmp2_result,
+                           const BasisSet& basis)
+    : rhf_(rhf_result), crmp2_(crmp2_result) {
+    // cholesky_rmp3.cc:18-29
+    nbf_ = basis.n_basis_functions();
+    nocc_ = crmp2_.n_occ;
+    nvirt_ = crmp2_.n_virt;
+    n_chol_vectors_ = crmp2_.n_chol_vectors;
+}
+
+RMP3Result CholeskyRMP3::compute() {
+    // cholesky_rmp3.cc:79-200: the three MO blocks of the vectors, t2(2) and
+    // E(3) on the device
+    if (!crmp2_.chol_device || crmp2_.n_basis != nbf_)
+        throw std::runtime_error("CholeskyRMP3: the CholeskyRMP2Result carries no Cholesky vectors for this basis");
+    const int n = nbf_, o = nocc_, v = nvirt_, Q = n_chol_vectors_;
+    const double* dL = crmp2_.chol_device->get();
+    const Mat& C = rhf_.C_alpha;
+    auto dCo = to_device(left_cols(C, o));
+    auto dCv = to_device(right_cols(C, v));
+    const Vec& eps = rhf_.orbital_energies_alpha;
+    auto deo = to_device(Vec(eps.begin(), eps.begin() + o));
+    auto dev = to_device(Vec(eps.begin() + o, eps.begin() + o + v));
+    device::Buffer B((size_t)Q * o * v), Loo((size_t)Q * o * o), Lvv((size_t)Q * v * v);
+    device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCv->get(), v, B.get()),
+                  "mshqc_hip_transform");
+    device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCo->get(), o, Loo.get()),
+                  "mshqc_hip_transform");
+    device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCv->get(), v, dCv->get(), v, Lvv.get()),
+                  "mshqc_hip_transform");
+    // first-order amplitudes: the RMP2 result's when it kept them
+    const size_t nt2 = (size_t)o * o * v * v;
+    device::Buffer t1(nt2);
+    if (crmp2_.t2.size() == nt2)
+        t1.upload(crmp2_.t2.data(), nt2);
+    else
+        device::check(mshqc_hip_rmp2_t2(device::ctx(), B.get(), Q, o, v, deo->get(), dev->get(), t1.get()),
+                      "mshqc_hip_rmp2_t2");
+    const bool keep = nt2 <= (size_t(1) << 27);
+    std::unique_ptr<device::Buffer> t22;
+    if (keep) t22 = std::make_unique<device::Buffer>(nt2);
+    double e3 = 0.0;
+    device::check(mshqc_hip_rmp3(device::ctx(), B.get(), Loo.get(), Lvv.get(), Q, o, v, t1.get(), deo->get(),
+                                 dev->get(), &e3, keep ? t22->get() : nullptr),
+                  "mshqc_hip_rmp3");
+    RMP3Result r;
+    r.e_rhf = crmp2_.e_rhf;
+    r.e_mp2 = crmp2_.e_corr;
+    r.e_mp3 = e3;
+    r.e_corr_total = r.e_mp2 + e3;
+    r.e_total = r.e_rhf + r.e_corr_total;
+    r.n_occ = o;
+    r.n_virt = v;
+    r.t2_1 = crmp2_.t2;
+    if (keep) {
+        r.t2_2.resize(nt2);
+        t22->download(r.t2_2.data(), nt2);
+    }
+    std::printf("[INFO] MP2 Correlation : %.8f Ha\n[INFO] MP3 Correction  : %.8f Ha\n"
+                "[INFO] Total Correlation: %.8f Ha\n[INFO] Total Energy    : %.8f Ha\n",
+                r.e_mp2, r.e_mp3, r.e_corr_total, r.e_total);
+    return r;
+}
+
+}  // namespace foundation
 
 // ================================================================ CD-UMP2
 

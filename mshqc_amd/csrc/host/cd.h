@@ -81,6 +81,8 @@ public:
     void set_vectors(const std::vector<Vec>& vectors, int n_basis);
     // Device copy [Q][n][n] (uploaded on first call).
     const double* device_vectors() const;
+    // The same device copy as a shared handle, for results that outlive this object.
+    std::shared_ptr<device::Buffer> device_buffer() const;
 
 private:
     std::vector<Vec> L_;
@@ -213,6 +215,11 @@ struct CholeskyRMP2Result {
     double e_rhf = 0.0, e_corr = 0.0, e_total = 0.0;
     // t2[i][j][a][b] (flattened, o*o*v*v) when stored; empty otherwise.
     Vec t2;
+    // The AO Cholesky vectors for consumers such as CholeskyRMP3: the
+    // reference copies them to the host here (cholesky_rmp2.cc:167); this
+    // result shares the device copy instead.
+    std::shared_ptr<device::Buffer> chol_device;
+    int n_basis = 0;
     int n_chol_vectors = 0;
     int n_occ = 0, n_virt = 0;
     double time_transform_s = 0.0, time_energy_s = 0.0;
@@ -245,6 +252,31 @@ private:
     const integrals::CholeskyERI* chol_ = nullptr;
     int nbf_, nocc_, nvirt_;
 };
+
+// ------------------------------------------------------------- CD-RMP3
+
+namespace foundation {
+// rmp3.h:52-65 of the reference (t2_1, t2_2 flattened [o][o][v][v]; t2_2
+// is kept when o^2 v^2 <= 2^27).
+struct RMP3Result {
+    double e_rhf = 0.0, e_mp2 = 0.0, e_mp3 = 0.0, e_corr_total = 0.0, e_total = 0.0;
+    int n_occ = 0, n_virt = 0;
+    Vec t2_1, t2_2;
+};
+
+// cholesky_rmp3.h / cholesky_rmp3.cc: second-order amplitudes and the MP3
+// energy from the CD-RMP2 result's vectors and amplitudes.
+class CholeskyRMP3 {
+public:
+    CholeskyRMP3(const SCFResult& rhf_result, const CholeskyRMP2Result& crmp2_result, const BasisSet& basis);
+    RMP3Result compute();
+
+private:
+    SCFResult rhf_;
+    CholeskyRMP2Result crmp2_;
+    int nbf_, nocc_, nvirt_, n_chol_vectors_;
+};
+}  // namespace foundation
 
 // ------------------------------------------------------------- CD-UMP2
 

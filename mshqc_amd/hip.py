@@ -38,6 +38,7 @@ _SIGS = {
     "mshqc_hip_jk_factor": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _vp],
     "mshqc_hip_jk_rohf": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp],
     "mshqc_hip_half_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp],
+    "mshqc_hip_rmp3": [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _D, _vp],
     "mshqc_hip_cholesky": [_vp, _vp, _c_int, ctypes.c_double, _c_int, _vp, ctypes.POINTER(_c_int),
                            ctypes.POINTER(_c_int), _D],
     "mshqc_hip_omp2_opdm": [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
@@ -313,6 +314,20 @@ class Context:
             E, *[self._ptr(g) for g in G])
         self._check(rc, "mshqc_hip_omp2_opdm")
         return tuple(E), tuple(G)
+
+    def rmp3(self, B, Loo, Lvv, t1, eps_o, eps_v, want_t2=False):
+        """(E3, t2_2 or None) of CD-RMP3 from B [Q,o,v], Loo [Q,o,o], Lvv [Q,v,v], t1 [o,o,v,v]."""
+        torch = self._torch
+        Q, o, v = B.shape
+        t22 = torch.empty((o, o, v, v), dtype=torch.float64, device=B.device) if want_t2 else None
+        E = ctypes.c_double()
+        self._follow_stream()
+        rc = lib().mshqc_hip_rmp3(self.h, self._ptr(B, name="B"), self._ptr(Loo, (Q, o, o), "Loo"),
+                                  self._ptr(Lvv, (Q, v, v), "Lvv"), Q, o, v, self._ptr(t1, (o, o, v, v), "t1"),
+                                  self._ptr(eps_o, (o,)), self._ptr(eps_v, (v,)), ctypes.byref(E),
+                                  self._ptr(t22) if want_t2 else None)
+        self._check(rc, "mshqc_hip_rmp3")
+        return E.value, t22
 
 
 _CTX = {}

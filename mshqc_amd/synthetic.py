@@ -18,6 +18,7 @@ Named workloads (sizes):
 ``ump2_o120``   Q=5600, o_a=120, v_a=1380, o_b=118, v_b=1382 (CD-UMP2)
 ``omp2_o40``    Q=1600, o_a=40, v_a=360, o_b=38, v_b=362  (CD-OMP2 amplitudes + OPDM)
 ``cd_n100``     ERI matrix M = L^T L from L [1500, 100*100] (pivoted Cholesky, thr 1e-6)
+``mp3_o20``     L: Q=800, n=200; o=20, v=180 (CD-RMP3 from the MO blocks)
 ==============  ==========================================================
 """
 from __future__ import annotations
@@ -32,6 +33,7 @@ WORKLOADS = {
     "ump2_o120": dict(kind="ump2", Q=5600, oa=120, va=1380, ob=118, vb=1382, seed=1005),
     "omp2_o40": dict(kind="omp2", Q=1600, oa=40, va=360, ob=38, vb=362, seed=1006),
     "cd_n100": dict(kind="cd", Q=1500, n=100, thr=1e-6, seed=1007),
+    "mp3_o20": dict(kind="rmp3", Q=800, n=200, o=20, seed=1008),
 }
 
 
@@ -96,6 +98,12 @@ def workload(name):
     w = WORKLOADS[name]
     if w["kind"] == "rmp2":
         return rmp2_inputs(w["Q"], w["o"], w["v"], w["seed"])
+    if w["kind"] == "rmp3":
+        rng = np.random.default_rng(w["seed"])
+        L = gen_L(w["Q"], w["n"], rng)
+        C = gen_Cocc(w["n"], w["n"], rng)
+        eo, ev = gen_eps(w["o"], w["n"] - w["o"], rng)
+        return L, C, eo, ev
     if w["kind"] == "cd":
         rng = np.random.default_rng(w["seed"])
         return gen_L(w["Q"], w["n"], rng).reshape(w["Q"], w["n"] * w["n"])

@@ -5,13 +5,14 @@
 //   CholeskyERI::decompose (src/integrals/cholesky_eri.cc)
 //   CholeskyRMP2::compute  (src/mp2/cholesky_rmp2.cc), reuse-vectors constructor
 //   CholeskyUMP2::compute  (src/mp2/cholesky_ump2.cc)
+//   CholeskyRMP3::compute  (src/mp3/cholesky_rmp3.cc), fed the RMP2 result
 // and prints the results as text.  Input layout (little-endian):
 //   int32 n, int32 n_occ_a, int32 n_occ_b; double threshold;
 //   double eri[n^4] ((i*n+j)*n+k)*n+l;
 //   double Ca[n*n], Cb[n*n] (row-major, columns = orbitals);
 //   double eps_a[n], eps_b[n]
 // Output lines: "nvec Q", "L <Q*n*n values>", "rmp2 e_corr", "t2 <o*o*v*v>",
-//   "ump2 ss_aa ss_bb os".
+//   "ump2 ss_aa ss_bb os", "rmp3 e_mp3", "t22 <o*o*v*v>".
 #include <cstdio>
 #include <cstdint>
 #include <iostream>
@@ -20,6 +21,7 @@
 
 #include "mshqc/basis.h"
 #include "mshqc/cholesky_rmp2.h"
+#include "mshqc/cholesky_rmp3.h"
 #include "mshqc/cholesky_ump2.h"
 #include "mshqc/integrals/cholesky_eri.h"
 #include "mshqc/molecule.h"
@@ -82,6 +84,9 @@ int main(int argc, char** argv) {
     uc.print_level = 0;
     CholeskyUMP2 ump2(scf, basis, nullptr, uc, chol);
     auto ur = ump2.compute();
+
+    foundation::CholeskyRMP3 rmp3(scf, rr, basis);
+    auto r3 = rmp3.compute();
     std::cout.rdbuf(keep);
 
     std::printf("nvec %d\n", chol.n_vectors());
@@ -96,5 +101,12 @@ int main(int argc, char** argv) {
             for (int a = 0; a < v; ++a)
                 for (int b = 0; b < v; ++b) std::printf(" %.17g", rr.t2(i, j, a, b));
     std::printf("\nump2 %.17g %.17g %.17g\n", ur.e_corr_ss_aa, ur.e_corr_ss_bb, ur.e_corr_os);
+    std::printf("rmp3 %.17g\n", r3.e_mp3);
+    std::printf("t22");
+    for (int i = 0; i < o; ++i)
+        for (int j = 0; j < o; ++j)
+            for (int a = 0; a < v; ++a)
+                for (int b = 0; b < v; ++b) std::printf(" %.17g", r3.t2_2(i, j, a, b));
+    std::printf("\n");
     return 0;
 }

@@ -1,0 +1,82 @@
+"""CD-RMP3 on the GPU against the oracle's restatement of
+cholesky_rmp3.cc:79-200 (itself pinned against the reference's compiled
+CholeskyRMP3), directly through the C ABI and through the drop-in classes."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import ROOT, check_close, to_dev, to_np
+from mshqc_amd import synthetic as syn
+from oracle import mp3_ref
+
+pytestmark = pytest.mark.gpu
+BD = os.path.join(ROOT, "mshqc_amd", "data", "basis")
+
+
+@pytest.mark.parametrize("Q,n,o,zblock,seed", [(23, 20, 4, None, 1), (31, 37, 6, None, 2),
+                                               (40, 70, 9, "3v3", 3), (17, 12, 1, None, 4)])
+def test_rmp3_matches_oracle(ctx, Q, n, o, zblock, seed):
+    rng = np.random.default_rng(seed)
+    L = syn.gen_L(Q, n, rng)
+    C = syn.gen_Cocc(n, n, rng)
+    v = n - o
+    eo, ev = syn.gen_eps(o, v, rng)
+    Co, Cv = C[:, :o].copy(), C[:, o:].copy()
+    B, Loo, Lvv = oracle.transform(L, Co, Cv), oracle.transform(L, Co, Co), oracle.transform(L, Cv, Cv)
+    _, t1 = oracle.rmp2_energy(B, eo, ev, want_t2=True)
+    E_ref, t22_ref = mp3_ref.rmp3(B, Loo, Lvv, t1, eo, ev)
+    print(f"rmp3 Q={Q} o={o} v={v} zblock={zblock}", flush=True)
+    if zblock:
+        ctx.set_zblock(3 * v ** 3)  # (ac|bd) in blocks of three a
+    try:
+        E, t22 = ctx.rmp3(*to_dev(B, Loo, Lvv, t1, eo, ev), want_t2=True)
+    finally:
+        ctx.set_zblock(1 << 27)
+    assert abs(E - E_ref) <= 1e-12 * abs(E_ref)
+    check_close(to_np(t22), t22_ref, 1e-12, "t2_2")
+
+
+@pytest.fixture(scope="module")
+def m(ctx):
+    from mshqc_amd import native_module
+
+    return native_module()
+
+
+def test_cd_rmp3_dropin(m):
+    """H4 chain, cc-pVDZ: CD-RHF -> CD-RMP2 -> CD-RMP3 through the classes,
+    against the oracle on the same vectors, orbitals and amplitudes; the same
+    energy when the RMP2 result kept no amplitudes (recomputed on the device)."""
+    mol = m.Molecule()
+    for k in range(4):
+        mol.add_atom(1, 0.0, 0.0, 1.4 * k)
+    b = m.BasisSet("cc-pvdz", mol, BD)
+    eng = m.IntegralEngine(mol, b)
+    chol = m.CholeskyERI(1e-6)
+    chol.decompose(eng.compute_eri())
+    n = b.n_basis_functions()
+    L = np.array(chol.get_L_vectors()).reshape(-1, n, n)
+    cfg = m.CholeskyRHFConfig()
+    cfg.print_level = 0
+    rhf = m.CholeskyRHF(mol, b, eng, cfg, chol).compute()
+    mc = m.CholeskyRMP2Config()
+    mc.print_level = 0
+    mp2 = m.CholeskyRMP2(mol, b, eng, rhf, mc, chol).compute()
+    r3 = m.CholeskyRMP3(rhf, mp2, b).compute()
+    o = mp2.n_occ
+    C, eps = rhf.C_alpha, rhf.orbital_energies_alpha
+    Co, Cv = C[:, :o], C[:, o:]
+    E_ref, t22_ref = mp3_ref.rmp3(oracle.transform(L, Co, Cv), oracle.transform(L, Co, Co),
+                                  oracle.transform(L, Cv, Cv), mp2.t2, eps[:o], eps[o:])
+    print(f"H4 CD-RMP3: {r3.e_mp3!r} oracle {E_ref!r}", flush=True)
+    assert abs(r3.e_mp3 - E_ref) <= 1e-12 * abs(E_ref)
+    check_close(r3.t2_2, t22_ref, 1e-12, "t2_2")
+    assert r3.e_mp2 == mp2.e_corr and r3.e_rhf == mp2.e_rhf
+    assert abs(r3.e_total - (mp2.e_rhf + mp2.e_corr + r3.e_mp3)) < 1e-12
+    mc.store_t2 = 0
+    mp2b = m.CholeskyRMP2(mol, b, eng, rhf, mc, chol).compute()
+    assert mp2b.t2 is None
+    r3b = m.CholeskyRMP3(rhf, mp2b, b).compute()
+    assert abs(r3b.e_mp3 - r3.e_mp3) <= 1e-12 * abs(r3.e_mp3)

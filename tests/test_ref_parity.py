@@ -1,6 +1,7 @@
 """The oracle against the reference's OWN code (oracle/_ref/ref_driver, built
 from the reference sources by oracle/ref.mk): pivoted Cholesky vectors,
-CD-RMP2 energy and amplitudes, CD-UMP2 energy components.  Skipped when the
+CD-RMP2 energy and amplitudes, CD-UMP2 energy components, CD-RMP3 energy and
+second-order amplitudes.  Skipped when the
 reference build is absent (it needs the reference checkout at build time)."""
 import os
 import struct
@@ -12,6 +13,7 @@ import pytest
 
 import oracle
 from conftest import ROOT
+from oracle import mp3_ref
 
 REF = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 BD = os.path.join(ROOT, "mshqc_amd", "data", "basis")
@@ -77,3 +79,8 @@ def test_oracle_matches_reference_build(problem, thr):
     Eu = oracle.ump2(Ba, Bb, ea[:na], ea[na:], eb[:nb], eb[nb:])
     for got, want in zip(ref["ump2"], Eu):
         assert abs(got - want) <= 1e-13 * max(abs(want), 1e-300) + 1e-17
+    # CD-RMP3 from the RMP2 amplitudes (alpha orbitals, n_occ = na)
+    Co, Cv = Ca[:, :na], Ca[:, na:]
+    E3, t22 = mp3_ref.rmp3(B, oracle.transform(L3, Co, Co), oracle.transform(L3, Cv, Cv), t2, ea[:na], ea[na:])
+    assert abs(ref["rmp3"][0] - E3) <= 1e-12 * abs(E3)
+    assert np.abs(ref["t22"] - t22.ravel()).max() <= 1e-12 * np.abs(t22).max()

@@ -246,6 +246,11 @@ __device__ __forceinline__ int lane_seg(int lane) { return (lane & 3) ^ ((lane >
 //   form G = Bl_i^T Br_j on the two 64-column halves, waves 2, 3 the mirrored
 //   H[a][b] = sum_Q Br^Q_ja Bl^Q_ib on the same halves; H is handed to the G
 //   waves through LDS for the epilogue.
+//   Symmetric modes, diagonal block k (encoded as unit (-1 - k, k)): rows and
+//   columns [128k, 128k + 128), tiles (2k + x, 2k + y), one wave per tile, G
+//   only; it replaces units (2k, k) and (2k + 1, k), whose tile (2k + 1, 2k)
+//   lies below the diagonal and whose diagonal tiles repeat their own mirror.
+//   The mirrors are all inside the block and go through LDS in the epilogue.
 //   Opposite-spin mode: unit (A2, B2) = 128 x 128, tiles (2A2 + x, 2B2 + y),
 //   one wave per tile, G only.
 // Stage = 16 Q of every panel row and slice, streamed by DMA into a
@@ -376,7 +381,8 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     const int i = ij.x, j = ij.y;
     const double w = p.pw[pslot];
     const int2 un = p.units[uslot];
-    const int R0 = SYM ? un.x * 64 : un.x * 128;  // first row (left virtual index)
+    const bool diag = SYM && un.x < 0;            // diagonal 128 x 128 block
+    const int R0 = diag ? un.y * 128 : SYM ? un.x * 64 : un.x * 128;  // first row (left virtual index)
     const int C0 = un.y * 128;                    // first column (right virtual index)
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -395,12 +401,20 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
         const int blk = BPW * wv + k;
         bool left;
         long r0;
-        if (SYM) {
+        if (SYM && !diag) {
             left = blk < 2 || blk >= 8;
             r0 = blk < 2   ? (long)i * p.vl + R0 + 32 * blk
                  : blk < 6 ? (long)j * p.vr + C0 + 32 * (blk - 2)
                  : blk < 8 ? (long)j * p.vr + R0 + 32 * (blk - 6)
                            : (long)i * p.vl + C0 + 32 * (blk - 8);
+        } else if (SYM) {
+            // diagonal block: 0-3 Bl_i rows, 4-7 Br_j rows; 8-11 repeat 0-3
+            // into LDS rows no wave reads, so every wave keeps the same load
+            // count as in the other units
+            left = blk < 4 || blk >= 8;
+            r0 = blk < 4   ? (long)i * p.vl + R0 + 32 * blk
+                 : blk < 8 ? (long)j * p.vr + C0 + 32 * (blk - 4)
+                           : (long)i * p.vl + R0 + 32 * (blk - 8);
         } else {
             left = blk < 4;
             r0 = blk < 4 ? (long)i * p.vl + R0 + 32 * blk : (long)j * p.vr + C0 + 32 * (blk - 4);
@@ -424,7 +438,7 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     int arow, bcol;          // first virtual index of this wave's rows / columns
     const int8_t *erow, *ecol;
     int vErow, vEcol;
-    if (SYM) {
+    if (SYM && !diag) {
         const int isH = wv >> 1, half = wv & 1;
         prow = isH ? 192 : 0;
         pcol = (isH ? 256 : 64) + 64 * half;
@@ -529,7 +543,43 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     }
 
     v4d accH[2][2][4];
-    if (SYM) {
+    if (SYM && diag) {
+        // Diagonal block: wave 2 holds tile (2k+1, 2k), the mirror of wave 1's
+        // tile (2k, 2k+1); waves 0 and 3 hold diagonal tiles, their own mirrors.
+        // Tiles go through LDS as [64][65] doubles (row, column) and the
+        // mirror is read transposed.
+        double* T = reinterpret_cast<double*>(lds);
+        constexpr int TP = 65;
+        const int put = wv == 2 ? 0 : wv == 0 ? 1 : wv == 3 ? 2 : -1;
+        if (put >= 0) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            T[(put * 64 + 32 * s + 8 * u + 4 * h + c) * TP + 32 * t + fr] = acc64[s][t][u][c];
+        }
+        __syncthreads();
+        if (wv == 2) return;
+        const int get = wv == 1 ? 0 : wv == 0 ? 1 : 2;
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        accH[s][t][u][c] = T[(get * 64 + 32 * t + fr) * TP + 32 * s + 8 * u + 4 * h + c];
+        const int A = 2 * un.y + (wv >> 1), B = 2 * un.y + (wv & 1);
+        if (A >= p.ntA || B >= p.ntB) return;
+        const long tslot = (long)A * p.ntA - (long)A * (A - 1) / 2 + (B - A);
+        pair_epilogue32<MODE>(p, i, j, w, 64 * A, 64 * B, A < B, lane, acc64, accH,
+                              (long)pslot * p.ntiles + tslot);
+    } else if (SYM) {
         // H waves hand their tile to the G waves through LDS (same lane map).
         double* Hs = reinterpret_cast<double*>(lds);
         const int isH = wv >> 1, half = wv & 1;
@@ -786,8 +836,10 @@ struct SliceCols {
 
 // Dynamic LDS of the pair kernel: two stages of every panel row and slice;
 // the symmetric kernel reuses it for the two 64 x 64 FP64 H tiles (64 KiB).
+// Dynamic LDS of the pair kernel: two stages; in the symmetric modes also the
+// epilogue exchanges (two H tiles, or three [64][65] tiles of a diagonal block).
 constexpr size_t pair_lds(int S, bool sym) {
-    return sym ? std::max((size_t)2 * S * 12 * 1024, (size_t)2 * 64 * 64 * 8) : (size_t)2 * S * 8 * 1024;
+    return sym ? std::max((size_t)2 * S * 12 * 1024, (size_t)3 * 64 * 65 * 8) : (size_t)2 * S * 8 * 1024;
 }
 
 template <int S>

@@ -129,12 +129,16 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
         for (int B = sym ? A : 0; B < ntB; ++B) tiles.push_back(make_int2(A, B));
     const int ntiles = (int)tiles.size();
     // Work units of the split-BF16 kernel: symmetric modes (A, B2) = tiles
-    // (A, 2B2), (A, 2B2+1) with A <= 2B2+1; opposite spin (A2, B2) = 2 x 2 tiles.
+    // (A, 2B2), (A, 2B2+1) with A < 2B2, and for each k the diagonal block
+    // (-1 - k, k) = tiles (2k .. 2k+1)^2 in place of units (2k, k) and
+    // (2k+1, k); opposite spin (A2, B2) = 2 x 2 tiles.
     std::vector<int2> units;
     if (sym) {
         for (int A = 0; A < ntA; ++A)
-            for (int B2 = 0; B2 < (ntA + 1) / 2; ++B2)
-                if (A <= 2 * B2 + 1) units.push_back(make_int2(A, B2));
+            for (int B2 = 0; B2 < (ntA + 1) / 2; ++B2) {
+                if (A == 2 * B2) units.push_back(make_int2(-1 - B2, B2));
+                else if (A < 2 * B2) units.push_back(make_int2(A, B2));
+            }
     } else {
         for (int A2 = 0; A2 < (ntA + 1) / 2; ++A2)
             for (int B2 = 0; B2 < (ntB + 1) / 2; ++B2) units.push_back(make_int2(A2, B2));

@@ -14,7 +14,7 @@ struct Bf16Operands {
     int slices = 0;   // slices of the (ia|jb) operands
     int gslices = 0;  // slices of the gradient-GEMM operands
     int Q = 0, Qp = 0, qblocks = 0;
-    int ol = 0, vl = 0, orr = 0, vr = 0, vlp = 0, vrp = 0;
+    int ol = 0, vl = 0, orr = 0, vr = 0;
     int vEl = 0, vEr = 0;  // row stride of the pair-kernel exponent tables
     // Pair-kernel operands: slices [S][Qp/16][o*v][16] (bf16 bits) and
     // exponents [Qp/128][o][vE], left and right (right = left in the
@@ -23,17 +23,24 @@ struct Bf16Operands {
     const int8_t* xl = nullptr;
     const uint16_t* sr = nullptr;
     const int8_t* xr = nullptr;
-    // Gradient-GEMM operands: slices [S][Q][o*vp] with exponents per
-    // (q, i, 128-wide a block), left and right.
+    // Gradient-GEMM operands, per occupied block of bo orbitals: block b
+    // holds columns (i, a) of its orbitals as one compact run of nb*v values,
+    // padded once to kw = round_up(nb*v, 128): slices [S][Q][kw] at
+    // g + b * S*Q*kwb and exponents [Q][kw/128] at ge + b * Q*kwb/128, with
+    // kwb the kw of a full block.  Left and right.
+    int bol = 0, bor = 0;
+    long kwbl = 0, kwbr = 0;
     const uint16_t* gl = nullptr;
     const int8_t* gel = nullptr;
     const uint16_t* gr = nullptr;
     const int8_t* ger = nullptr;
 };
 
-// Slice Bl (and Br when not null) into BF16 planes; `grad` adds the GEMM layout.
+// Slice Bl (and Br when not null) into BF16 planes; `grad` adds the GEMM
+// layout for occupied blocks of bo_l (left) and bo_r (right) orbitals.
 int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int vl,
-                 const double* Br, int orr, int vr, int Q, bool grad, Bf16Operands* out);
+                 const double* Br, int orr, int vr, int Q, bool grad, int bo_l, int bo_r,
+                 Bf16Operands* out);
 // Pair-tile kernel on BF16 MFMA with the shared FP64 epilogue.
 int bf16_launch_pair(mshqc_hip_ctx* ctx, int mode, const PairArgs& a, const Bf16Operands& ops);
 // Gradient contractions of one occupied block pair from the FP64 amplitude

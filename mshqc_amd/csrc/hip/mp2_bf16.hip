@@ -916,7 +916,7 @@ struct AllowLdsGemm {
 }  // namespace
 
 int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int vl, const double* Br,
-                 int orr, int vr, int Q, bool grad, Bf16Operands* out) {
+                 int orr, int vr, int Q, bool grad, int bo_l, int bo_r, Bf16Operands* out) {
     const int S = slices;
     by_slices<AllowLds>(S, ctx);
     Bf16Operands o;
@@ -949,25 +949,35 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
         if (rc) return rc;
     }
     if (grad) {
-        // GEMM-operand slices of B: rows q, columns (i, a) blocked by 128 along a.
-        o.vlp = round_up(vl, GB);
-        o.vrp = round_up(vr, GB);
+        // GEMM-operand slices of B per occupied block: rows q, the block's
+        // columns (i, a) as one run, 128-wide exponent blocks along it.
         const int SG = o.gslices;
-        const size_t gl = (size_t)SG * Q * ol * o.vlp * 2, gr = Br ? (size_t)SG * Q * orr * o.vrp * 2 : 0;
-        const size_t gel = (size_t)Q * ol * (o.vlp / GB), ger = Br ? (size_t)Q * orr * (o.vrp / GB) : 0;
+        o.bol = std::max(1, std::min(bo_l, ol));
+        o.bor = Br ? std::max(1, std::min(bo_r, orr)) : o.bol;
+        o.kwbl = round_up(o.bol * vl, GB);
+        o.kwbr = Br ? round_up(o.bor * vr, GB) : o.kwbl;
+        const int nbl = ceil_div(ol, o.bol), nbr = Br ? ceil_div(orr, o.bor) : 0;
+        const size_t gl = (size_t)nbl * SG * Q * o.kwbl * 2, gr = (size_t)nbr * SG * Q * o.kwbr * 2;
+        const size_t gel = (size_t)nbl * Q * (o.kwbl / GB), ger = (size_t)nbr * Q * (o.kwbr / GB);
         char* g = static_cast<char*>(workspace(ctx, WS_SLICE2, gl + gr + gel + ger + 1024));
         if (!g) return MSHQC_HIP_ERR_ALLOC;
         o.gl = reinterpret_cast<uint16_t*>(g);
         o.gr = Br ? reinterpret_cast<uint16_t*>(g + gl) : o.gl;
         o.gel = reinterpret_cast<int8_t*>(g + gl + gr);
         o.ger = Br ? reinterpret_cast<int8_t*>(g + gl + gr + gel) : o.gel;
-        rc = by_gslices<SliceRows>(SG, ctx, Bl, (long)ovl, Q, ol, vl, o.vlp, const_cast<uint16_t*>(o.gl),
-                                  const_cast<int8_t*>(o.gel));
-        if (rc) return rc;
-        if (Br) {
-            rc = by_gslices<SliceRows>(SG, ctx, Br, (long)ovr, Q, orr, vr, o.vrp, const_cast<uint16_t*>(o.gr),
-                                      const_cast<int8_t*>(o.ger));
-            if (rc) return rc;
+        for (int side = 0; side < (Br ? 2 : 1); ++side) {
+            const double* src = side ? Br : Bl;
+            const int oo = side ? orr : ol, vv = side ? vr : vl, bo = side ? o.bor : o.bol;
+            const long kwb = side ? o.kwbr : o.kwbl;
+            uint16_t* dst = const_cast<uint16_t*>(side ? o.gr : o.gl);
+            int8_t* ex = const_cast<int8_t*>(side ? o.ger : o.gel);
+            for (int b = 0, i0 = 0; i0 < oo; ++b, i0 += bo) {
+                const int nb = std::min(bo, oo - i0);
+                rc = by_gslices<SliceRows>(SG, ctx, src + (long)i0 * vv, (long)oo * vv, Q, 1, nb * vv,
+                                          round_up(nb * vv, GB), dst + (size_t)b * SG * Q * kwb,
+                                          ex + (size_t)b * Q * (kwb / GB));
+                if (rc) return rc;
+            }
         }
     }
     o.ol = ol;
@@ -986,37 +996,42 @@ int bf16_launch_pair(mshqc_hip_ctx* ctx, int mode, const PairArgs& a, const Bf16
 int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& o, const double* Z, long ldz, int I0,
                        int nI, int J0, int nJ, double* dBl, double* dBr, bool second) {
     const int S = o.gslices, Q = o.Q;
-    const int vl = o.vl, vr = o.vr, vlp = o.vlp, vrp = o.vrp;
+    const int vl = o.vl, vr = o.vr;
+    if (I0 % o.bol != 0 || J0 % o.bor != 0 || nI > o.bol || nJ > o.bor)
+        return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 gradient: block (%d+%d, %d+%d) off the %d/%d grid",
+                         I0, nI, J0, nJ, o.bol, o.bor);
     int rc;
     if (dBl) {
         // dBl[q][(i,a)] += sum_{(j,b)} Br[q][(j,b)] Z[(i,a)][(j,b)]
-        const long R = (long)nI * vl, kw = (long)nJ * vrp;
-        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * nJ * (vrp / GB);
+        const long R = (long)nI * vl, kw = round_up(nJ * vr, GB);
+        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * (kw / GB);
         char* w = static_cast<char*>(workspace(ctx, WS_ZS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;
         uint16_t* zs = reinterpret_cast<uint16_t*>(w);
         int8_t* zx = reinterpret_cast<int8_t*>(w + zb);
-        rc = by_gslices<SliceRows>(S, ctx, Z, ldz, (int)R, nJ, vr, vrp, zs, zx);
+        rc = by_gslices<SliceRows>(S, ctx, Z, ldz, (int)R, 1, nJ * vr, (int)kw, zs, zx);
         if (rc) return rc;
-        SlicedMat A{o.gr + (long)J0 * vrp, (long)Q * o.orr * vrp, (long)o.orr * vrp,
-                    o.ger + (long)J0 * (vrp / GB), (long)o.orr * (vrp / GB), Q};
-        SlicedMat X{zs, (long)R * kw, kw, zx, (long)nJ * (vrp / GB), (int)R};
+        const long b = J0 / o.bor;
+        SlicedMat A{o.gr + (size_t)b * S * Q * o.kwbr, (long)Q * kw, kw, o.ger + (size_t)b * Q * (o.kwbr / GB),
+                    kw / GB, Q};
+        SlicedMat X{zs, (long)R * kw, kw, zx, kw / GB, (int)R};
         rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBl + (long)I0 * vl, (long)o.ol * vl);
         if (rc) return rc;
     }
     if (second && dBr) {
         // dBr[q][(j,b)] += sum_{(i,a)} Bl[q][(i,a)] Z[(i,a)][(j,b)]
-        const long R = (long)nJ * vr, kw = (long)nI * vlp;
-        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * nI * (vlp / GB);
+        const long R = (long)nJ * vr, kw = round_up(nI * vl, GB);
+        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * (kw / GB);
         char* w = static_cast<char*>(workspace(ctx, WS_ZTS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;
         uint16_t* zs = reinterpret_cast<uint16_t*>(w);
         int8_t* zx = reinterpret_cast<int8_t*>(w + zb);
-        rc = by_gslices<SliceCols>(S, ctx, Z, ldz, nI, vl, vlp, (int)R, zs, zx);
+        rc = by_gslices<SliceCols>(S, ctx, Z, ldz, 1, nI * vl, (int)kw, (int)R, zs, zx);
         if (rc) return rc;
-        SlicedMat A{o.gl + (long)I0 * vlp, (long)Q * o.ol * vlp, (long)o.ol * vlp,
-                    o.gel + (long)I0 * (vlp / GB), (long)o.ol * (vlp / GB), Q};
-        SlicedMat X{zs, (long)R * kw, kw, zx, (long)nI * (vlp / GB), (int)R};
+        const long b = I0 / o.bol;
+        SlicedMat A{o.gl + (size_t)b * S * Q * o.kwbl, (long)Q * kw, kw, o.gel + (size_t)b * Q * (o.kwbl / GB),
+                    kw / GB, Q};
+        SlicedMat X{zs, (long)R * kw, kw, zx, kw / GB, (int)R};
         rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBr + (long)J0 * vr, (long)o.orr * vr);
         if (rc) return rc;
     }

@@ -248,7 +248,7 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
     if (pr.precision != 0) {
         int rc = bf16_prepare(ctx, pr.precision, pr.Bl, pr.ol, pr.vl,
                               sym ? nullptr : pr.Br, pr.orr, pr.vr, pr.Q,
-                              pr.grad && (pr.dBl || pr.dBr), &bops);
+                              pr.grad && (pr.dBl || pr.dBr), bo_l, bo_r, &bops);
         if (rc) return rc;
     }
 

@@ -250,7 +250,8 @@ __device__ __forceinline__ int lane_seg(int lane) { return (lane & 3) ^ ((lane >
 //   columns [128k, 128k + 128), tiles (2k + x, 2k + y), one wave per tile, G
 //   only; it replaces units (2k, k) and (2k + 1, k), whose tile (2k + 1, 2k)
 //   lies below the diagonal and whose diagonal tiles repeat their own mirror.
-//   The mirrors are all inside the block and go through LDS in the epilogue.
+//   The mirrors are all inside the block and go through LDS in the epilogue;
+//   its stages hold 8 blocks (0-3 Bl_i, 4-7 Br_j), two per wave.
 //   Opposite-spin mode: unit (A2, B2) = 128 x 128, tiles (2A2 + x, 2B2 + y),
 //   one wave per tile, G only.
 // Stage = 16 Q of every panel row and slice, streamed by DMA into a
@@ -396,9 +397,12 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     const int lseg = (lane & 1) ^ (((lr >> 2) ^ (lr >> 3)) & 1);
     const uint16_t* rp[BPW];
     long rplane[BPW], rstep[BPW];
+    int rdst[BPW];
 #pragma unroll
     for (int k = 0; k < BPW; ++k) {
-        const int blk = BPW * wv + k;
+        // a diagonal block streams its 8 blocks two per wave
+        const int blk = diag ? (k < 2 ? 2 * wv + k : 2 * wv) : BPW * wv + k;
+        rdst[k] = blk * 1024;
         bool left;
         long r0;
         if (SYM && !diag) {
@@ -408,13 +412,9 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
                  : blk < 8 ? (long)j * p.vr + R0 + 32 * (blk - 6)
                            : (long)i * p.vl + C0 + 32 * (blk - 8);
         } else if (SYM) {
-            // diagonal block: 0-3 Bl_i rows, 4-7 Br_j rows; 8-11 repeat 0-3
-            // into LDS rows no wave reads, so every wave keeps the same load
-            // count as in the other units
-            left = blk < 4 || blk >= 8;
-            r0 = blk < 4   ? (long)i * p.vl + R0 + 32 * blk
-                 : blk < 8 ? (long)j * p.vr + C0 + 32 * (blk - 4)
-                           : (long)i * p.vl + R0 + 32 * (blk - 8);
+            // diagonal block: 0-3 Bl_i rows, 4-7 Br_j rows
+            left = blk < 4;
+            r0 = blk < 4 ? (long)i * p.vl + R0 + 32 * blk : (long)j * p.vr + C0 + 32 * (blk - 4);
         } else {
             left = blk < 4;
             r0 = blk < 4 ? (long)i * p.vl + R0 + 32 * blk : (long)j * p.vr + C0 + 32 * (blk - 4);
@@ -427,10 +427,14 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     }
     // One slice of a stage; a stage's loads are spread over its MFMA groups
     // (a burst of them stalls instruction issue behind the address unit).
+    // (Every stage barrier waits for all of a wave's loads, so the
+    // diagonal block's lower per-wave load count needs no other change.)
     auto issue_slice = [&](int st, int buf, int sl) {
 #pragma unroll
-        for (int k = 0; k < BPW; ++k)
-            dma16(rp[k] + sl * rplane[k] + st * rstep[k], lds + buf * STAGE + sl * SLICE + (BPW * wv + k) * 1024);
+        for (int k = 0; k < BPW; ++k) {
+            if (SYM && diag && k >= 2) continue;
+            dma16(rp[k] + sl * rplane[k] + st * rstep[k], lds + buf * STAGE + sl * SLICE + rdst[k]);
+        }
     };
 
     // This wave's operand rows in a stage (LDS panel rows) and exponent rows.

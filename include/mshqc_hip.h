@@ -262,6 +262,24 @@ int mshqc_hip_rmp3(mshqc_hip_ctx* ctx, const double* B, const double* Loo,
                    const double* Lvv, int Q, int o, int v, const double* t1,
                    const double* eps_o, const double* eps_v, double* E3, double* t2_2);
 
+/*
+ * CD-UMP3 energy components E = {ss_aa, ss_bb, os} (HOST, 3), replacing
+ * CholeskyUMP3::compute_t2_first_order and compute_mp3_ss_alpha / _beta /
+ * _os (src/mp3/cholesky_ump3.cc:212-637), from per-spin B [Q][o][v],
+ * Loo [Q][o][o], Lvv [Q][v][v] and orbital energies.  The reference builds
+ * its first-order amplitudes (and the beta ring term) from its o x v blocks
+ * read in column-major order as if row-major, i.e. from
+ * B~[q][i*v + a] = B[q][(r mod o)*v + r div o] at r = i*v + a; this call
+ * reproduces that.  A spin with no occupied or no virtual orbitals
+ * contributes 0.  Workspace: a few (o v)^2 and o^2 v^2 blocks plus (ac|bd)
+ * blocks bounded by the mshqc_hip_set_zblock cap.
+ */
+int mshqc_hip_ump3(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
+                   const double* Looa, const double* Loob, const double* Lvva,
+                   const double* Lvvb, int Q, int oa, int va, int ob, int vb,
+                   const double* eps_oa, const double* eps_va,
+                   const double* eps_ob, const double* eps_vb, double* E);
+
 #ifdef __cplusplus
 }
 #endif

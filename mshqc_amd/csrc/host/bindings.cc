@@ -295,6 +295,32 @@ PYBIND11_MODULE(_mshqc, m) {
              py::arg("config"), py::arg("existing_cholesky"))
         .def("compute", &CholeskyROHF::compute);
 
+    // ---------------------------------------------------------- UMP3
+    // (no Python binding in the reference; names follow cholesky_ump3.h)
+    py::class_<CholeskyUMP3Config>(m, "CholeskyUMP3Config")
+        .def(py::init<>())
+        .def_readwrite("cholesky_threshold", &CholeskyUMP3Config::cholesky_threshold)
+        .def_readwrite("use_ump2_result", &CholeskyUMP3Config::use_ump2_result)
+        .def_readwrite("store_intermediates", &CholeskyUMP3Config::store_intermediates)
+        .def_readwrite("validate_energy", &CholeskyUMP3Config::validate_energy)
+        .def_readwrite("print_level", &CholeskyUMP3Config::print_level);
+    py::class_<CholeskyUMP3Result>(m, "CholeskyUMP3Result")
+        .def_readonly("e_mp2_total", &CholeskyUMP3Result::e_mp2_total)
+        .def_readonly("e_mp3_ss_aa", &CholeskyUMP3Result::e_mp3_ss_aa)
+        .def_readonly("e_mp3_ss_bb", &CholeskyUMP3Result::e_mp3_ss_bb)
+        .def_readonly("e_mp3_os", &CholeskyUMP3Result::e_mp3_os)
+        .def_readonly("e_mp3_total", &CholeskyUMP3Result::e_mp3_total)
+        .def_readonly("e_corr_total", &CholeskyUMP3Result::e_corr_total)
+        .def_readonly("e_total", &CholeskyUMP3Result::e_total)
+        .def_readonly("n_cholesky_vectors", &CholeskyUMP3Result::n_cholesky_vectors)
+        .def_readonly("time_mp3_s", &CholeskyUMP3Result::time_mp3_s);
+    py::class_<CholeskyUMP3>(m, "CholeskyUMP3")
+        .def(py::init<const SCFResult&, const BasisSet&, std::shared_ptr<IntegralEngine>, const CholeskyUMP3Config&>(),
+             py::arg("uhf_result"), py::arg("basis"), py::arg("integrals"), py::arg("config") = CholeskyUMP3Config())
+        .def(py::init<const CholeskyUMP2&, const CholeskyUMP3Config&>(), py::arg("ump2_solver"),
+             py::arg("config") = CholeskyUMP3Config())
+        .def("compute", &CholeskyUMP3::compute);
+
     // ---------------------------------------------------------- OMP2
     // bindings.cc:311-331 (OMP2Result) and 495-519 of the reference
     py::class_<OMP2Result> omp2r(m, "OMP2Result");

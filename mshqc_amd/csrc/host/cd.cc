@@ -848,6 +848,88 @@ CholeskyUMP2Result CholeskyUMP2::compute() {
     return r;
 }
 
+// ================================================================ CD-UMP3
+
+CholeskyUMP3::CholeskyUMP3(const SCFResult& uhf_result, const BasisSet& basis,
+                           std::shared_ptr<IntegralEngine> integrals, const CholeskyUMP3Config& config)
+    : uhf_(uhf_result), basis_(basis), integrals_(std::move(integrals)), config_(config),
+      chol_(config.cholesky_threshold) {
+    // cholesky_ump3.cc:26-39
+    nbf_ = basis.n_basis_functions();
+    na_ = uhf_.n_occ_alpha;
+    nb_ = uhf_.n_occ_beta;
+    va_ = nbf_ - na_;
+    vb_ = nbf_ - nb_;
+}
+
+CholeskyUMP3::CholeskyUMP3(const CholeskyUMP2& ump2_solver, const CholeskyUMP3Config& config)
+    : uhf_(ump2_solver.uhf_), basis_(ump2_solver.basis_), integrals_(ump2_solver.integrals_), config_(config),
+      chol_(ump2_solver.get_cholesky()) {
+    // cholesky_ump3.cc:41-56: reuses the UMP2 solver's vectors
+    nbf_ = ump2_solver.nbf_;
+    na_ = ump2_solver.na_;
+    nb_ = ump2_solver.nb_;
+    va_ = ump2_solver.va_;
+    vb_ = ump2_solver.vb_;
+}
+
+CholeskyUMP3Result CholeskyUMP3::compute() {
+    // cholesky_ump3.cc:160-205
+    CholeskyUMP3Result r;
+    auto t_start = std::chrono::high_resolution_clock::now();
+    if (config_.print_level > 0)
+        std::printf("\n========================================\n  Cholesky UMP3 (Block-Contraction Optimized)\n"
+                    "========================================\n");
+    if (chol_.n_vectors() == 0) chol_.decompose_gpu(integrals_->compute_eri());
+    if (chol_.n_basis() != nbf_) throw std::runtime_error("CholeskyUMP3: Cholesky vectors do not match the basis");
+    const int n = nbf_, Q = chol_.n_vectors();
+    r.n_cholesky_vectors = Q;
+    r.compression_ratio = chol_.compression_ratio();
+    const double* dL = chol_.device_vectors();
+    // MO blocks per spin (:112-150): B = C_occ^T L C_vir, Loo, Lvv
+    struct Blocks {
+        std::unique_ptr<device::Buffer> B, Loo, Lvv, eo, ev;
+    } blk[2];
+    for (int s = 0; s < 2; ++s) {
+        const Mat& C = s == 0 ? uhf_.C_alpha : uhf_.C_beta;
+        const Vec& e = s == 0 ? uhf_.orbital_energies_alpha : uhf_.orbital_energies_beta;
+        const int o = s == 0 ? na_ : nb_, v = s == 0 ? va_ : vb_;
+        if (o == 0 || v == 0) continue;
+        auto dCo = to_device(left_cols(C, o));
+        auto dCv = to_device(right_cols(C, v));
+        Blocks& b = blk[s];
+        b.B = std::make_unique<device::Buffer>((size_t)Q * o * v);
+        b.Loo = std::make_unique<device::Buffer>((size_t)Q * o * o);
+        b.Lvv = std::make_unique<device::Buffer>((size_t)Q * v * v);
+        device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCv->get(), v, b.B->get()),
+                      "mshqc_hip_transform");
+        device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCo->get(), o, b.Loo->get()),
+                      "mshqc_hip_transform");
+        device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCv->get(), v, dCv->get(), v, b.Lvv->get()),
+                      "mshqc_hip_transform");
+        b.eo = to_device(Vec(e.begin(), e.begin() + o));
+        b.ev = to_device(Vec(e.begin() + o, e.begin() + o + v));
+    }
+    auto ptr = [](const std::unique_ptr<device::Buffer>& x) -> const double* { return x ? x->get() : nullptr; };
+    auto t1 = std::chrono::high_resolution_clock::now();
+    double E[3] = {0, 0, 0};
+    device::check(mshqc_hip_ump3(device::ctx(), ptr(blk[0].B), ptr(blk[1].B), ptr(blk[0].Loo), ptr(blk[1].Loo),
+                                 ptr(blk[0].Lvv), ptr(blk[1].Lvv), Q, na_, va_, nb_, vb_, ptr(blk[0].eo),
+                                 ptr(blk[0].ev), ptr(blk[1].eo), ptr(blk[1].ev), E),
+                  "mshqc_hip_ump3");
+    r.time_mp3_s = seconds_since(t1);
+    r.e_mp3_ss_aa = E[0];
+    r.e_mp3_ss_bb = E[1];
+    r.e_mp3_os = E[2];
+    r.e_mp3_total = r.e_mp3_ss_aa + r.e_mp3_ss_bb + r.e_mp3_os;
+    r.e_corr_total = r.e_mp2_total + r.e_mp3_total;
+    r.time_total_s = seconds_since(t_start);
+    if (config_.print_level > 0)
+        std::printf("MP3 Corr (aa): %.8f Ha\nMP3 Corr (bb): %.8f Ha\nMP3 Corr (ab): %.8f Ha\nTotal MP3 Corr: %.8f Ha\n",
+                    r.e_mp3_ss_aa, r.e_mp3_ss_bb, r.e_mp3_os, r.e_mp3_total);
+    return r;
+}
+
 // ================================================================ CD-OMP2
 
 CholeskyOMP2::CholeskyOMP2(const Molecule& mol, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,

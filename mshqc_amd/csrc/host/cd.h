@@ -304,11 +304,51 @@ public:
     const integrals::CholeskyERI& get_cholesky() const { return chol_; }
 
 private:
+    friend class CholeskyUMP3;
     SCFResult uhf_;
     BasisSet basis_;
     std::shared_ptr<IntegralEngine> integrals_;
     CholeskyUMP2Config config_;
     integrals::CholeskyERI chol_;  // shares the device copy with the source
+    int nbf_, na_, nb_, va_, vb_;
+};
+
+// ------------------------------------------------------------- CD-UMP3
+
+struct CholeskyUMP3Config {
+    double cholesky_threshold = 1e-6;
+    bool use_ump2_result = true;
+    bool store_intermediates = false;
+    bool validate_energy = false;
+    int print_level = 1;
+};
+
+// cholesky_ump3.h of the reference; like its compute(), e_mp2_* stay 0 and
+// e_total is not filled.
+struct CholeskyUMP3Result {
+    double e_mp2_ss_aa = 0.0, e_mp2_ss_bb = 0.0, e_mp2_os = 0.0, e_mp2_total = 0.0;
+    double e_mp3_ss_aa = 0.0, e_mp3_ss_bb = 0.0, e_mp3_os = 0.0, e_mp3_total = 0.0;
+    double e_corr_total = 0.0, e_total = 0.0;
+    int n_cholesky_vectors = 0;
+    double compression_ratio = 0.0, memory_mb = 0.0;
+    double time_mp2_s = 0.0, time_mp3_s = 0.0, time_total_s = 0.0;
+};
+
+// cholesky_ump3.cc: MP3 energy components from the UHF orbitals and the
+// Cholesky vectors (decomposed here, or reused from a CholeskyUMP2 solver).
+class CholeskyUMP3 {
+public:
+    CholeskyUMP3(const SCFResult& uhf_result, const BasisSet& basis, std::shared_ptr<IntegralEngine> integrals,
+                 const CholeskyUMP3Config& config = CholeskyUMP3Config());
+    CholeskyUMP3(const CholeskyUMP2& ump2_solver, const CholeskyUMP3Config& config = CholeskyUMP3Config());
+    CholeskyUMP3Result compute();
+
+private:
+    SCFResult uhf_;
+    BasisSet basis_;
+    std::shared_ptr<IntegralEngine> integrals_;
+    CholeskyUMP3Config config_;
+    integrals::CholeskyERI chol_;
     int nbf_, na_, nb_, va_, vb_;
 };
 

@@ -38,6 +38,8 @@ _SIGS = {
     "mshqc_hip_jk_factor": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _vp],
     "mshqc_hip_jk_rohf": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp],
     "mshqc_hip_half_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp],
+    "mshqc_hip_ump3": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp,
+                       _vp, _vp, _D],
     "mshqc_hip_rmp3": [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp, _D, _vp],
     "mshqc_hip_cholesky": [_vp, _vp, _c_int, ctypes.c_double, _c_int, _vp, ctypes.POINTER(_c_int),
                            ctypes.POINTER(_c_int), _D],
@@ -328,6 +330,20 @@ class Context:
                                   self._ptr(t22) if want_t2 else None)
         self._check(rc, "mshqc_hip_rmp3")
         return E.value, t22
+
+    def ump3(self, Ba, Bb, Looa, Loob, Lvva, Lvvb, eoa, eva, eob, evb):
+        """(E_ss_aa, E_ss_bb, E_os) of CD-UMP3 from per-spin MO blocks."""
+        Q, oa, va = Ba.shape
+        _, ob, vb = Bb.shape
+        E = (ctypes.c_double * 3)()
+        self._follow_stream()
+        rc = lib().mshqc_hip_ump3(
+            self.h, self._ptr(Ba, name="Ba"), self._ptr(Bb, (Q, ob, vb), "Bb"), self._ptr(Looa, (Q, oa, oa), "Looa"),
+            self._ptr(Loob, (Q, ob, ob), "Loob"), self._ptr(Lvva, (Q, va, va), "Lvva"),
+            self._ptr(Lvvb, (Q, vb, vb), "Lvvb"), Q, oa, va, ob, vb, self._ptr(eoa, (oa,)), self._ptr(eva, (va,)),
+            self._ptr(eob, (ob,)), self._ptr(evb, (vb,)), E)
+        self._check(rc, "mshqc_hip_ump3")
+        return tuple(E)
 
 
 _CTX = {}

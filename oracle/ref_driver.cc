@@ -6,13 +6,14 @@
 //   CholeskyRMP2::compute  (src/mp2/cholesky_rmp2.cc), reuse-vectors constructor
 //   CholeskyUMP2::compute  (src/mp2/cholesky_ump2.cc)
 //   CholeskyRMP3::compute  (src/mp3/cholesky_rmp3.cc), fed the RMP2 result
+//   CholeskyUMP3::compute  (src/mp3/cholesky_ump3.cc), from the UMP2 solver
 // and prints the results as text.  Input layout (little-endian):
 //   int32 n, int32 n_occ_a, int32 n_occ_b; double threshold;
 //   double eri[n^4] ((i*n+j)*n+k)*n+l;
 //   double Ca[n*n], Cb[n*n] (row-major, columns = orbitals);
 //   double eps_a[n], eps_b[n]
 // Output lines: "nvec Q", "L <Q*n*n values>", "rmp2 e_corr", "t2 <o*o*v*v>",
-//   "ump2 ss_aa ss_bb os", "rmp3 e_mp3", "t22 <o*o*v*v>".
+//   "ump2 ss_aa ss_bb os", "rmp3 e_mp3", "t22 <o*o*v*v>", "ump3 aa bb os".
 #include <cstdio>
 #include <cstdint>
 #include <iostream>
@@ -22,6 +23,7 @@
 #include "mshqc/basis.h"
 #include "mshqc/cholesky_rmp2.h"
 #include "mshqc/cholesky_rmp3.h"
+#include "mshqc/cholesky_ump3.h"
 #include "mshqc/cholesky_ump2.h"
 #include "mshqc/integrals/cholesky_eri.h"
 #include "mshqc/molecule.h"
@@ -87,6 +89,11 @@ int main(int argc, char** argv) {
 
     foundation::CholeskyRMP3 rmp3(scf, rr, basis);
     auto r3 = rmp3.compute();
+
+    CholeskyUMP3Config u3c;
+    u3c.print_level = 0;
+    CholeskyUMP3 ump3(ump2, u3c);
+    auto u3 = ump3.compute();
     std::cout.rdbuf(keep);
 
     std::printf("nvec %d\n", chol.n_vectors());
@@ -107,6 +114,6 @@ int main(int argc, char** argv) {
         for (int j = 0; j < o; ++j)
             for (int a = 0; a < v; ++a)
                 for (int b = 0; b < v; ++b) std::printf(" %.17g", r3.t2_2(i, j, a, b));
-    std::printf("\n");
+    std::printf("\nump3 %.17g %.17g %.17g\n", u3.e_mp3_ss_aa, u3.e_mp3_ss_bb, u3.e_mp3_os);
     return 0;
 }

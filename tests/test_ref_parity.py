@@ -1,7 +1,7 @@
 """The oracle against the reference's OWN code (oracle/_ref/ref_driver, built
 from the reference sources by oracle/ref.mk): pivoted Cholesky vectors,
 CD-RMP2 energy and amplitudes, CD-UMP2 energy components, CD-RMP3 energy and
-second-order amplitudes.  Skipped when the
+second-order amplitudes, CD-UMP3 energy components.  Skipped when the
 reference build is absent (it needs the reference checkout at build time)."""
 import os
 import struct
@@ -84,3 +84,10 @@ def test_oracle_matches_reference_build(problem, thr):
     E3, t22 = mp3_ref.rmp3(B, oracle.transform(L3, Co, Co), oracle.transform(L3, Cv, Cv), t2, ea[:na], ea[na:])
     assert abs(ref["rmp3"][0] - E3) <= 1e-12 * abs(E3)
     assert np.abs(ref["t22"] - t22.ravel()).max() <= 1e-12 * np.abs(t22).max()
+    # CD-UMP3 components (alpha: na occupied of Ca, beta: nb of Cb)
+    Cbo, Cbv = Cb[:, :nb], Cb[:, nb:]
+    E3u = mp3_ref.ump3(Ba, Bb, oracle.transform(L3, Co, Co), oracle.transform(L3, Cbo, Cbo),
+                       oracle.transform(L3, Cv, Cv), oracle.transform(L3, Cbv, Cbv),
+                       ea[:na], ea[na:], eb[:nb], eb[nb:])
+    for got, want in zip(ref["ump3"], E3u):
+        assert abs(got - want) <= 1e-12 * abs(want)

@@ -403,6 +403,55 @@ def run_rmp3(args, torch, hip, syn, w):
     return line
 
 
+def ump3_flops(Q, oa, va, ob, vb):
+    """FP64 flops of the CD-UMP3 call.  Per spin: (ia|jb) unscrambled and
+    scrambled (4 (ov)^2 Q), (kl|ij) and the hole-hole term (2 o^4 (Q + v^2)),
+    the (ab|cd) ladder through the factors and (kj|bc) (6 o^2 v^2 Q), the ring
+    product (2 (ov)^3).  Opposite spin: (ia|jb) (2 oa va ob vb Q), (ki|lj) and
+    its term (2 (oa ob)^2 (Q + va vb)), (ac|bd) and its ladder
+    (2 (va vb)^2 (Q + oa ob)), the two rings (2 oa va ob vb (oa va + ob vb))."""
+    f = 0.0
+    for o, v in ((oa, va), (ob, vb)):
+        f += 4.0 * (o * v) ** 2 * Q + 2.0 * o ** 4 * (Q + v * v) + 6.0 * (o * v) ** 2 * Q + 2.0 * (o * v) ** 3
+    ova, ovb = oa * va, ob * vb
+    f += 2.0 * ova * ovb * Q + 2.0 * (oa * ob) ** 2 * (Q + va * vb) + 2.0 * (va * vb) ** 2 * (Q + oa * ob) \
+        + 2.0 * ova * ovb * (ova + ovb)
+    return f
+
+
+def run_ump3(args, torch, hip, syn, w):
+    """CD-UMP3 energy components (cholesky_ump3.cc:160-637) from device MO
+    blocks of both spins; the transforms are made before timing."""
+    ctx = hip.context(0)
+    Q, n, oa, ob = w["Q"], w["n"], w["oa"], w["ob"]
+    va, vb = n - oa, n - ob
+    log(f"[bench] generating {args.workload}: Q={Q} n={n} oa={oa} va={va} ob={ob} vb={vb}")
+    L, spa, spb = syn.workload(args.workload)
+    dL = torch.from_numpy(L).cuda()
+    del L
+    blocks = []
+    for (C, eo, ev), o in ((spa, oa), (spb, ob)):
+        Co = torch.from_numpy(np.ascontiguousarray(C[:, :o])).cuda()
+        Cv = torch.from_numpy(np.ascontiguousarray(C[:, o:])).cuda()
+        blocks.append((ctx.transform(dL, Co, Cv), ctx.transform(dL, Co, Co), ctx.transform(dL, Cv, Cv),
+                       torch.from_numpy(eo).cuda(), torch.from_numpy(ev).cuda()))
+    (Ba, Looa, Lvva, eoa, eva), (Bb, Loob, Lvvb, eob, evb) = blocks
+    sec, out, prof = timed(torch, ctx, lambda: ctx.ump3(Ba, Bb, Looa, Loob, Lvva, Lvvb, eoa, eva, eob, evb),
+                           args.steps, args.warmup)
+    flop = ump3_flops(Q, oa, va, ob, vb)
+    log(f"[bench] ump3: {sec * 1e3:.1f} ms/step, {flop / sec / 1e12:.1f} TFLOP/s FP64, E3 = {list(out)!r}, "
+        f"GEMMs {prof[1]}")
+    config = {"workload": f"{args.workload}: CD-UMP3 energy components", "Q": Q, "n": n, "oa": oa, "va": va,
+              "ob": ob, "vb": vb, "parallelism": "single GPU"}
+    line = base_line(args, sec, "f64", config, flop)
+    achieved = flop / sec / 1e12
+    line["roofline"] = {"kernel": "FP64 MFMA GEMMs ((ac|bd) build, ladders, rings)", "bound": "mfma",
+                        "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s", "frac": achieved / 78.6,
+                        "traffic": None}
+    line["energy_components"] = list(out)
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -437,6 +486,8 @@ def main():
         line = run_cd(args, torch, hip, syn, w)
     elif w["kind"] == "rmp3":
         line = run_rmp3(args, torch, hip, syn, w)
+    elif w["kind"] == "ump3":
+        line = run_ump3(args, torch, hip, syn, w)
     else:
         line = run_jk(args, torch, hip, syn, w)
     print(json.dumps(line), flush=True)

@@ -19,6 +19,7 @@ Named workloads (sizes):
 ``omp2_o40``    Q=1600, o_a=40, v_a=360, o_b=38, v_b=362  (CD-OMP2 amplitudes + OPDM)
 ``cd_n100``     ERI matrix M = L^T L from L [1500, 100*100] (pivoted Cholesky, thr 1e-6)
 ``mp3_o20``     L: Q=800, n=200; o=20, v=180 (CD-RMP3 from the MO blocks)
+``ump3_o20``    L: Q=800, n=200; o_a=20, o_b=18 (CD-UMP3 from the MO blocks)
 ==============  ==========================================================
 """
 from __future__ import annotations
@@ -34,6 +35,7 @@ WORKLOADS = {
     "omp2_o40": dict(kind="omp2", Q=1600, oa=40, va=360, ob=38, vb=362, seed=1006),
     "cd_n100": dict(kind="cd", Q=1500, n=100, thr=1e-6, seed=1007),
     "mp3_o20": dict(kind="rmp3", Q=800, n=200, o=20, seed=1008),
+    "ump3_o20": dict(kind="ump3", Q=800, n=200, oa=20, ob=18, seed=1009),
 }
 
 
@@ -104,6 +106,14 @@ def workload(name):
         C = gen_Cocc(w["n"], w["n"], rng)
         eo, ev = gen_eps(w["o"], w["n"] - w["o"], rng)
         return L, C, eo, ev
+    if w["kind"] == "ump3":
+        rng = np.random.default_rng(w["seed"])
+        n = w["n"]
+        L = gen_L(w["Q"], n, rng)
+        Ca, Cb = gen_Cocc(n, n, rng), gen_Cocc(n, n, rng)
+        eoa, eva = gen_eps(w["oa"], n - w["oa"], rng)
+        eob, evb = gen_eps(w["ob"], n - w["ob"], rng)
+        return L, (Ca, eoa, eva), (Cb, eob, evb)
     if w["kind"] == "cd":
         rng = np.random.default_rng(w["seed"])
         return gen_L(w["Q"], w["n"], rng).reshape(w["Q"], w["n"] * w["n"])

@@ -200,6 +200,8 @@ def mp2_line(args, results, precs, W3, config, energy_of, grads_of):
 
 def run_mp2(args, torch, hip, syn, w):
     ctx = hip.context(0)
+    if args.zblock:
+        ctx.set_zblock(int(args.zblock))
     precs = [parse_precision(args.precision)]
     if args.compare:
         precs.append(parse_precision(args.compare))
@@ -321,7 +323,9 @@ def run_omp2(args, torch, hip, syn, w):
     (Ba, eoa, eva), (Bb, eob, evb) = syn.workload(args.workload)
     dev = [torch.from_numpy(np.ascontiguousarray(x)).cuda() for x in (Ba, Bb, eoa, eva, eob, evb)]
     sec, out, prof = timed(torch, ctx, lambda: ctx.omp2_opdm(*dev), args.steps, args.warmup)
-    swapped = 1 << 27 < oa * va * vb * ob
+    # the opposite-spin pass is repeated with the spins swapped when its
+    # amplitude block (oa va vb per beta index) cannot hold every beta index
+    swapped = hip.ZBLOCK_DEFAULT // (oa * va * vb) < ob
     flop = omp2_flops(Q, oa, va, ob, vb, swapped)
     E = out[0]
     log(f"[bench] omp2_opdm: {sec * 1e3:.1f} ms/step, {flop / sec / 1e12:.1f} TFLOP/s FP64, "
@@ -461,6 +465,9 @@ def main():
     ap.add_argument("--compare", default="fp64", help="second precision timed beside it ('' for none)")
     ap.add_argument("--workload", default="mp2_o100")
     ap.add_argument("--cpu-pairs", type=int, default=16)
+    ap.add_argument("--zblock", type=float, default=0,
+                    help="cap (doubles) of the gradient amplitude block per occupied block pair "
+                         "(0: the library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB moved by one plain device kernel before timing (a known-size "

@@ -62,9 +62,9 @@ int mshqc_hip_set_stream(mshqc_hip_ctx* ctx, void* stream);
 /* Device time, in ms, of the last compute call (HIP events around its kernels). */
 int mshqc_hip_last_ms(const mshqc_hip_ctx* ctx, float* ms);
 /* Cap, in doubles, of the per-occupied-block amplitude workspace used by the
- * gradient calls and by mshqc_hip_omp2_opdm (default 2^27 = 1 GiB).  Smaller
- * caps mean more, smaller occupied blocks; results do not depend on it beyond
- * rounding. */
+ * gradient calls and by mshqc_hip_omp2_opdm, and of the (ac|bd) blocks of the
+ * MP3 calls (default 2^30 = 8 GiB; 0 restores the default).  Smaller caps mean
+ * more, smaller blocks; results do not depend on it beyond rounding. */
 int mshqc_hip_set_zblock(mshqc_hip_ctx* ctx, size_t elems);
 /* Per-kernel-class timing (HIP events around each launch, on the stream the
  * kernel runs on).  kind: 0 = pair-block (ia|jb) kernels, 1 = GEMM kernels

@@ -14,6 +14,9 @@
 
 // ---------------------------------------------------------------- context
 
+// Default cap, in doubles, of the per-block amplitude workspaces (8 GiB).
+constexpr size_t ZBLOCK_DEFAULT = size_t(1) << 30;
+
 struct mshqc_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;     // stream all work is queued on
@@ -29,7 +32,7 @@ struct mshqc_hip_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.f;
     // Zero-block sizing knob for the gradient workspace (elements of Z).
-    size_t zblock_elems = size_t(1) << 27;
+    size_t zblock_elems = ZBLOCK_DEFAULT;
     // Per-kernel-class timing.
     bool prof = false;
     struct Pending {

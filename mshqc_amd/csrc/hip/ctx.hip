@@ -147,8 +147,8 @@ int mshqc_hip_last_ms(const mshqc_hip_ctx* ctx, float* ms) {
 }
 
 int mshqc_hip_set_zblock(mshqc_hip_ctx* ctx, size_t elems) {
-    if (!ctx || elems < 1) return MSHQC_HIP_ERR_ARG;
-    ctx->zblock_elems = elems;
+    if (!ctx) return MSHQC_HIP_ERR_ARG;
+    ctx->zblock_elems = elems ? elems : ZBLOCK_DEFAULT;
     return MSHQC_HIP_OK;
 }
 

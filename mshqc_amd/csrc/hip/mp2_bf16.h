@@ -25,8 +25,8 @@ struct Bf16Operands {
     const int8_t* xr = nullptr;
     // Gradient-GEMM operands, per occupied block of bo orbitals: block b
     // holds columns (i, a) of its orbitals as one compact run of nb*v values,
-    // padded once to kw = round_up(nb*v, 128): slices [S][Q][kw] at
-    // g + b * S*Q*kwb and exponents [Q][kw/128] at ge + b * Q*kwb/128, with
+    // padded once to kw = round_up(nb*v, 256): slices [S][Q][kw] at
+    // g + b * S*Q*kwb and exponents [Q][kw/256] at ge + b * Q*kwb/256, with
     // kwb the kw of a full block.  Left and right.
     int bol = 0, bor = 0;
     long kwbl = 0, kwbr = 0;

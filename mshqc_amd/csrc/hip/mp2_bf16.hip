@@ -9,7 +9,7 @@
 //   slice_rows_kernel / slice_cols_kernel
 //                       row- and column-blocked slices of the gradient GEMM
 //                       operands (B and the amplitude block Z, Z^T), one
-//                       exponent per row and 128-wide k block
+//                       exponent per row and 256-wide k block
 //   gemm_bf16_nt_kernel C[m][n] += sum_k A[m][k] X[n][k] from slices
 // The arithmetic contract is described in ozaki.h.
 #include <algorithm>
@@ -30,7 +30,12 @@ constexpr int KS = 16;    // Q per pair-kernel stage (one 32x32x16 MFMA K step)
 constexpr int RB = 32;    // LDS bytes per panel row, slice and pair-kernel stage
 constexpr int KC = 32;    // contraction chunk of the gradient GEMM (one 16x16x32 MFMA K step)
 constexpr int LROW = 64;  // LDS bytes per 32-element BF16 row
-constexpr int GB = 128;   // exponent block of the gradient-GEMM operands along k
+// Exponent block of the gradient-GEMM operands along k.  Its tier-0 sums
+// reach 2^24, where the MFMA accumulation can lose a last bit (2^-24 of the
+// block sum); with three slices the truncation already sits at that level,
+// so the 256-wide block (half the FP64 flushes of 128) keeps the gradient
+// well inside its bar.
+constexpr int GB = 256;
 constexpr int GT = 128;   // gradient-GEMM workgroup tile (rows and columns)
 
 // LDS byte offset of 16-byte segment `sg` of panel row `row`: the segment is
@@ -112,7 +117,7 @@ __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restric
 
 // src[R][ld] with column index seg*v + b (seg < nseg, b < v)  ->
 // dst[S][R][nseg*vp] (column seg*vp + b, zero padded), ex[R][nseg*(vp/GB)].
-// One wave per (row, seg, GB-wide b tile); each lane slices two elements.
+// One wave per (row, seg, GB-wide b tile); each lane slices GB/64 elements.
 template <int S>
 __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restrict__ src, long ld, int R,
                                                          int nseg, int v, int vp,
@@ -126,23 +131,27 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
     const int row = (int)(job / (nseg * ntv));
     const int rem = (int)(job % (nseg * ntv));
     const int seg = rem / ntv, tile = rem % ntv;
-    const int b0 = tile * GB + lane, b1 = b0 + 64;
+    constexpr int NE = GB / 64;
     const double* s = src + (long)row * ld + (long)seg * v;
-    const double x0 = b0 < v ? s[b0] : 0.0;
-    const double x1 = b1 < v ? s[b1] : 0.0;
-    double mx = fmax(fabs(x0), fabs(x1));
+    double x[NE];
+    double mx = 0.0;
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        const int b = tile * GB + 64 * u + lane;
+        x[u] = b < v ? s[b] : 0.0;
+        mx = fmax(mx, fabs(x[u]));
+    }
     for (int m = 1; m < 64; m <<= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
     const int e = block_exponent(mx);
-    uint16_t s0[S], s1[S];
-    slice_value<S>(x0, e, s0);
-    slice_value<S>(x1, e, s1);
     const long kw = (long)nseg * vp;
     const long slice = (long)R * kw;
     uint16_t* d = dst + (long)row * kw + (long)seg * vp + tile * GB + lane;
 #pragma unroll
-    for (int p = 0; p < S; ++p) {
-        d[p * slice] = s0[p];
-        d[p * slice + 64] = s1[p];
+    for (int u = 0; u < NE; ++u) {
+        uint16_t sv[S];
+        slice_value<S>(x[u], e, sv);
+#pragma unroll
+        for (int p = 0; p < S; ++p) d[p * slice + 64 * u] = sv[p];
     }
     if (lane == 0) ex[(long)row * nseg * ntv + rem] = (int8_t)e;
 }
@@ -150,49 +159,52 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
 // Transposed slicing of Z[(i',a)][(j',b)] (rows i'*v + a, i' < nI; ld = ldz;
 // nc = nJ*vr columns):  dst[S][nc][nI*vp] with row n = column of Z and
 // column i'*vp + a;  ex[nc][nI*(vp/GB)].  A workgroup transposes a tile of
-// 32 columns x GB values of a through LDS.
+// SC_COLS columns x GB values of a through LDS.
+constexpr int SC_COLS = 16;
+
 template <int S>
 __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restrict__ Z, long ldz, int nI,
                                                          int v, int vp, int nc,
                                                          uint16_t* __restrict__ dst,
                                                          int8_t* __restrict__ ex) {
-    __shared__ double T[GB][33];
-    __shared__ int exs[32];
+    constexpr int NC = SC_COLS, PARTS = 256 / NC, PER = GB / PARTS;  // PER values of a per thread
+    static_assert(PER % 8 == 0, "whole 16-byte stores");
+    __shared__ double T[GB][NC + 1];
+    __shared__ int exs[NC];
     const int ntv = vp / GB;
-    const int c0 = blockIdx.x * 32;
+    const int c0 = blockIdx.x * NC;
     const int ip = blockIdx.y / ntv, at = blockIdx.y % ntv, a0 = at * GB;
     const int tid = threadIdx.x;
     {
-        const int cc = tid & 31;
-        for (int k = 0; k < GB / 8; ++k) {
-            const int aa = (tid >> 5) + 8 * k;
+        const int cc = tid % NC;
+        for (int k = 0; k < GB / PARTS; ++k) {
+            const int aa = tid / NC + PARTS * k;
             double x = 0.0;
             if (a0 + aa < v && c0 + cc < nc) x = Z[((long)ip * v + a0 + aa) * ldz + c0 + cc];
             T[aa][cc] = x;
         }
     }
     __syncthreads();
+    // thread -> (column cc, PER consecutive a)
+    const int cc = tid / PARTS, part = tid % PARTS;
     {
-        // 8 lanes per column, 16 values each, then a shuffle maximum.
-        const int cc = tid >> 3, part = tid & 7;
+        // PARTS lanes per column, then a shuffle maximum.
         double mx = 0.0;
-        for (int k = 0; k < GB / 8; ++k) mx = fmax(mx, fabs(T[part * (GB / 8) + k][cc]));
-        for (int m = 1; m < 8; m <<= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
+        for (int k = 0; k < PER; ++k) mx = fmax(mx, fabs(T[part * PER + k][cc]));
+        for (int m = 1; m < PARTS; m <<= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
         if (part == 0) exs[cc] = block_exponent(mx);
     }
     __syncthreads();
     const long kw = (long)nI * vp, slice = (long)nc * kw;
-    // thread -> (column cc, 16 consecutive a)
-    const int cc = tid >> 3, part = tid & 7;
     if (c0 + cc >= nc) return;
     const int e = exs[cc];
-    uint16_t* d = dst + (long)(c0 + cc) * kw + (long)ip * vp + a0 + 16 * part;
-    for (int u8 = 0; u8 < 16; u8 += 8) {
+    uint16_t* d = dst + (long)(c0 + cc) * kw + (long)ip * vp + a0 + PER * part;
+    for (int u8 = 0; u8 < PER; u8 += 8) {
         uint16_t out[S][8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             uint16_t m[S];
-            slice_value<S>(T[16 * part + u8 + u][cc], e, m);
+            slice_value<S>(T[PER * part + u8 + u][cc], e, m);
 #pragma unroll
             for (int p = 0; p < S; ++p) out[p][u] = m[p];
         }
@@ -644,9 +656,9 @@ struct SlicedMat {
 // LDS by the DMA path, two stages ahead (one barrier per stage, no staging
 // registers).  The
 // gradient needs ~1e-7 relative accuracy, so products go to two FP32 tiers:
-// c = 0 (exact integers) and c >= 1 (rounding near 2^-31 of the block
-// scale); both are flushed to FP64 with the block exponents once per GB-wide
-// k block.
+// c = 0 (integers, exact up to the last bit above 2^23) and c >= 1 (rounding
+// near 2^-31 of the block scale); both are flushed to FP64 with the block
+// exponents once per GB-wide k block.
 template <int S>
 __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMat X, int K, double* __restrict__ C,
                                                            long ldc) {
@@ -831,7 +843,7 @@ template <int S>
 struct SliceCols {
     static int run(mshqc_hip_ctx* ctx, const double* Z, long ldz, int nI, int v, int vp, int nc,
                    uint16_t* dst, int8_t* ex) {
-        dim3 grid(ceil_div(nc, 32), nI * (vp / GB));
+        dim3 grid(ceil_div(nc, SC_COLS), nI * (vp / GB));
         slice_cols_kernel<S><<<grid, 256, 0, ctx->stream>>>(Z, ldz, nI, v, vp, nc, dst, ex);
         LAUNCH_CHECK(ctx);
         return 0;
@@ -954,7 +966,7 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     }
     if (grad) {
         // GEMM-operand slices of B per occupied block: rows q, the block's
-        // columns (i, a) as one run, 128-wide exponent blocks along it.
+        // columns (i, a) as one run, GB-wide exponent blocks along it.
         const int SG = o.gslices;
         o.bol = std::max(1, std::min(bo_l, ol));
         o.bor = Br ? std::max(1, std::min(bo_r, orr)) : o.bol;

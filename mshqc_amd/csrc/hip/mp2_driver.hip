@@ -173,10 +173,16 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
     } else {
         const int vmax = std::max(pr.vl, pr.vr);
         const double side = std::sqrt((double)ctx->zblock_elems);
-        int bo = (int)std::floor(side / vmax);
-        bo = std::max(1, bo);
-        bo_l = std::min(bo, pr.ol);
-        bo_r = std::min(bo, pr.orr);
+        const int bo = std::max(1, (int)std::floor(side / vmax));
+        // as few blocks as a cap of bo orbitals allows, each only as large as
+        // that count needs (o = 100, bo = 29: four blocks of 25, not 29 x 3 + 13)
+        auto balanced = [bo](int o) {
+            if (o <= 0) return 0;
+            const int nb = ceil_div(o, std::min(bo, o));
+            return ceil_div(o, nb);
+        };
+        bo_l = balanced(pr.ol);
+        bo_r = balanced(pr.orr);
         if (sym) bo_r = bo_l;
         for (int I0 = 0; I0 < pr.ol; I0 += bo_l) {
             const int nI = std::min(bo_l, pr.ol - I0);

@@ -8,11 +8,13 @@
 // |m_0| <= 256 and |m_p| <= 128 (p >= 1); each s_p is therefore exact in
 // BF16.  A slice product s_p s_q = m_p m_q 2^{-8(p+q)} is exact.  Products
 // are summed on the FP32 MFMA accumulator in three tiers by class c = p+q;
-// for blocks of 128 products per output (128-wide Q or k blocks):
+// for blocks of 128 products per output (the pair kernel's 128-wide Q blocks):
 //   tier 0 (c = 0): integers <= 2^16 each, sums <= 2^23          -> exact
 //   tier 1 (c = 1): multiples of 2^-8, <= 2^7 each, sums <= 2^15 -> exact
 // (with 256-wide blocks, tier-0 sums up to 2^24 measurably lost their last
-// bit on gfx950 MFMAs, so blocks stay at 128)
+// bit on gfx950 MFMAs, so the pair kernel's blocks stay at 128; the
+// three-slice gradient GEMM, whose truncation sits at 2^-24 anyway, uses
+// 256-wide k blocks and two tiers, c = 0 and c >= 1)
 //   tier 2 (c >= 2): below 2^-1 per product; FP32 rounding there sits near
 //                    2^-40 of the tier-0 scale, under the truncation error.
 // After every block the tiers are added in FP64 and scaled by 2^{eA+eB-16}.

@@ -55,6 +55,9 @@ _SIGS = {
     + [_vp] * 6,
 }
 
+#: Default cap (doubles) of the per-block amplitude workspaces of a context.
+ZBLOCK_DEFAULT = 1 << 30
+
 #: Every symbol include/mshqc_hip.h declares.
 EXPORTED = sorted(list(_SIGS) + ["mshqc_hip_last_error", "mshqc_hip_version"])
 
@@ -137,7 +140,8 @@ class Context:
         return _vp(t.data_ptr())
 
     def set_zblock(self, elems: int):
-        """Cap (doubles) of the gradient amplitude workspace per occupied block."""
+        """Cap (doubles) of the per-block amplitude workspaces; 0 restores
+        the default (ZBLOCK_DEFAULT)."""
         self._check(lib().mshqc_hip_set_zblock(self.h, int(elems)), "mshqc_hip_set_zblock")
 
     def profile(self, enable: bool = True):

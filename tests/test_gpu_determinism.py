@@ -28,7 +28,7 @@ def test_rmp2_grad_bitwise_repeatable(ctx, precision, shape):
     try:
         runs = [ctx.rmp2_grad(dB, deo, dev, precision=precision) for _ in range(2)]
     finally:
-        ctx.set_zblock(1 << 27)
+        ctx.set_zblock(0)
     (E1, g1, o1, v1), (E2, g2, o2, v2) = runs
     print(f"precision {precision}: E {E1!r} {E2!r}", flush=True)
     assert E1 == E2
@@ -45,7 +45,7 @@ def test_ump2_grad_bitwise_repeatable(ctx, precision):
     try:
         runs = [ctx.ump2_grad(*args, precision=precision) for _ in range(2)]
     finally:
-        ctx.set_zblock(1 << 27)
+        ctx.set_zblock(0)
     (E1, g1), (E2, g2) = runs
     assert E1 == E2
     for a, b in zip(g1, g2):

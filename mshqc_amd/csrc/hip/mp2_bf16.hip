@@ -38,6 +38,15 @@ constexpr int LROW = 64;  // LDS bytes per 32-element BF16 row
 constexpr int GB = 256;
 constexpr int GT = 128;   // gradient-GEMM workgroup tile (rows and columns)
 
+// Slots of the pair kernel's LDS ring of slices (12 KiB per slice in the
+// symmetric modes, 8 KiB in the opposite-spin mode) within 160 KiB: two whole
+// stages plus as many slices of a third as fit (all S where possible).
+constexpr int ring_slots(int S, bool sym) {
+    const int fit = (160 * 1024) / ((sym ? 12 : 8) * 1024);
+    const int third = fit - 2 * S < S ? fit - 2 * S : S;
+    return 2 * S + third;
+}
+
 // LDS byte offset of 16-byte segment `sg` of panel row `row`: the segment is
 // XOR-swizzled by (row >> 1) & 3, which makes the 16-lane fragment reads and
 // the 4-lanes-per-row stores free of bank conflicts (measured on gfx950).
@@ -266,8 +275,9 @@ __device__ __forceinline__ int lane_seg(int lane) { return (lane & 3) ^ ((lane >
 //   its stages hold 8 blocks (0-3 Bl_i, 4-7 Br_j), two per wave.
 //   Opposite-spin mode: unit (A2, B2) = 128 x 128, tiles (2A2 + x, 2B2 + y),
 //   one wave per tile, G only.
-// Stage = 16 Q of every panel row and slice, streamed by DMA into a
-// double-buffered LDS.  Panel rows (32-row blocks): symmetric: 0-1 Bl_i[A],
+// Stage = 16 Q of every panel row and slice, streamed by DMA into LDS
+// (double-buffered in the symmetric modes, triple-buffered in the opposite-
+// spin mode).  Panel rows (32-row blocks): symmetric: 0-1 Bl_i[A],
 // 2-5 Br_j[B], 6-7 Br_j[A], 8-11 Bl_i[B]; opposite spin: 0-3 Bl_i[A],
 // 4-7 Br_j[B].  Each wave streams 3 (2) of the blocks for every slice.
 //
@@ -286,10 +296,13 @@ __device__ __forceinline__ v16f mfma32_bf16(const v8s& a, const v8s& b, v16f c) 
 // modes) at row a = a0 + 32s + 8u + 4h + c, column b = b0 + 32t + l%32,
 // h = l/32.  Writes the weighted tile partials to `slot` and, when requested,
 // the amplitudes z (gradient) or t2.
+// The orbital energies and the output come in as restrict parameters: the
+// amplitude stores then do not order the later energy loads behind them.
 template <int MODE>
-__device__ __forceinline__ void pair_epilogue32(const PairArgs& p, int i, int j, double w, int a0, int b0,
-                                                bool mirror, int lane, const v4d (&G)[2][2][4],
-                                                const v4d (&H)[2][2][4], long slot) {
+__device__ __forceinline__ void pair_epilogue32_r(const PairArgs& p, const double* __restrict__ evl,
+                                                  const double* __restrict__ evr, double* __restrict__ Z,
+                                                  int i, int j, double w, int a0, int b0, bool mirror, int lane,
+                                                  const v4d (&G)[2][2][4], const v4d (&H)[2][2][4], long slot) {
 #pragma clang fp contract(off)
     const bool grad = p.s_part != nullptr;
     const int h = lane >> 5, col = lane & 31;
@@ -305,23 +318,34 @@ __device__ __forceinline__ void pair_epilogue32(const PairArgs& p, int i, int j,
 #pragma unroll
             for (int c = 0; c < 4; ++c) rowq[s][u][c] = 0.0;
     double* t2base = nullptr;
-    if (MODE == MODE_T2) t2base = p.Z + ((long)i * p.orr + j) * p.vl * p.vr;
+    if (MODE == MODE_T2) t2base = Z + ((long)i * p.orr + j) * p.vl * p.vr;
     const long zr = (long)(i - p.zi0) * p.vl, zc = (long)(j - p.zj0) * p.vr;
     const long zr2 = (long)(j - p.zi0) * p.vl, zc2 = (long)(i - p.zj0) * p.vr;
+    // The lane's virtual energies, loaded up front with clamped indices so
+    // the loads issue together instead of one masked load-and-wait each.
+    double evbv[2], eva[2][4][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) evbv[t] = evr[min(b0 + 32 * t + col, p.vr - 1)];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) eva[s][u][c] = evl[min(a0 + 32 * s + 8 * u + 4 * h + c, p.vl - 1)];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int b = b0 + 32 * t + col;
-        if (b >= p.vr) continue;
-        const double evb = p.evr[b];
+        const bool bin = b < p.vr;
+        const double evb = evbv[t];
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < 4; ++u) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const int a = a0 + 32 * s + 8 * u + 4 * h + c;
-                    if (a >= p.vl) continue;
-                    const double D = ei - p.evl[a] - evb;
+                    if (!bin || a >= p.vl) continue;
+                    const double D = ei - eva[s][u][c] - evb;
                     const double g = G[s][t][u][c];
                     const double hh = (MODE == MODE_OS) ? 0.0 : H[s][t][u][c];
                     double e1, m1, z1;
@@ -342,15 +366,16 @@ __device__ __forceinline__ void pair_epilogue32(const PairArgs& p, int i, int j,
                         const bool ok = fabs(D) >= 1e-12;
                         t2base[(long)a * p.vr + b] = ok ? g / D : 0.0;
                         if (mirror) t2base[(long)b * p.vr + a] = ok ? hh / D : 0.0;
-                    } else if (p.Z) {
-                        p.Z[(zr + a) * p.ldz + zc + b] = z1;
-                        if (mirror) p.Z[(zr + b) * p.ldz + zc + a] = z2;
+                    } else if (Z) {
+                        Z[(zr + a) * p.ldz + zc + b] = z1;
+                        if (mirror) Z[(zr + b) * p.ldz + zc + a] = z2;
                         if (p.ztrans && i != j) {
-                            p.Z[(zr2 + b) * p.ldz + zc2 + a] = z1;
-                            if (mirror) p.Z[(zr2 + a) * p.ldz + zc2 + b] = z2;
+                            Z[(zr2 + b) * p.ldz + zc2 + a] = z1;
+                            if (mirror) Z[(zr2 + a) * p.ldz + zc2 + b] = z2;
                         }
                     }
                 }
+            }
     }
     // Deterministic reductions: fixed in-lane order, then fixed shuffle trees.
     e = xor_sum(e, 1, 64);
@@ -374,6 +399,13 @@ __device__ __forceinline__ void pair_epilogue32(const PairArgs& p, int i, int j,
     }
 }
 
+template <int MODE>
+__device__ __forceinline__ void pair_epilogue32(const PairArgs& p, int i, int j, double w, int a0, int b0,
+                                                bool mirror, int lane, const v4d (&G)[2][2][4],
+                                                const v4d (&H)[2][2][4], long slot) {
+    pair_epilogue32_r<MODE>(p, p.evl, p.evr, p.Z, i, j, w, a0, b0, mirror, lane, G, H, slot);
+}
+
 template <int MODE, int S>
 __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16_t* __restrict__ Psl,
                                                         const int8_t* __restrict__ Pel,
@@ -384,7 +416,16 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     constexpr int NBLK = SYM ? 12 : 8;         // 32-row panel blocks per slice and stage
     constexpr int BPW = NBLK / 4;              // blocks streamed per wave
     constexpr long SLICE = (long)NBLK * 1024;  // bytes per slice and stage
-    constexpr long STAGE = S * SLICE;
+    // LDS ring of per-slice slots: (stage st, slice sl) lives in slot
+    // (S st + sl) mod NSLOT, NSLOT = 2S + PF.  During stage st a wave loads
+    // slices PF..S-1 of stage st+1, then slices 0..PF-1 of stage st+2; those
+    // land in exactly the S slots stage st-1 used, free after its closing
+    // barrier, and the barrier closing stage st waits only until the stage
+    // st+1 slices have landed (the stage st+2 ones may stay in flight).
+    // PF = S (three whole stages) where LDS allows; the symmetric units'
+    // 12 KiB slices fit 13 slots, so PF = 3 at S = 5.
+    constexpr int NSLOT = ring_slots(S, SYM);
+    constexpr int PF = NSLOT - 2 * S;  // slices of stage st+2 loaded during stage st
     extern __shared__ __align__(16) char lds[];
 
     const long work = xcd_work_index();
@@ -439,14 +480,21 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     }
     // One slice of a stage; a stage's loads are spread over its MFMA groups
     // (a burst of them stalls instruction issue behind the address unit).
-    // (Every stage barrier waits for all of a wave's loads, so the
-    // diagonal block's lower per-wave load count needs no other change.)
-    auto issue_slice = [&](int st, int buf, int sl) {
+    auto slot = [&](int st, int sl) { return lds + (long)((S * st + sl) % NSLOT) * SLICE; };
+    auto issue_slice = [&](int st, int sl) {
+        char* dst = slot(st, sl);
 #pragma unroll
         for (int k = 0; k < BPW; ++k) {
             if (SYM && diag && k >= 2) continue;
-            dma16(rp[k] + sl * rplane[k] + st * rstep[k], lds + buf * STAGE + sl * SLICE + rdst[k]);
+            dma16(rp[k] + sl * rplane[k] + st * rstep[k], dst + rdst[k]);
         }
+    };
+    // Wait until at most the PF newest slices' loads of this wave are in
+    // flight (rows are clamped, so every wave of a unit issues the same
+    // number of loads per slice: BPW, or 2 in a diagonal block).
+    auto ring_barrier = [&]() {
+        if (SYM && diag) dma_barrier<PF * 2>();
+        else dma_barrier<PF * BPW>();
     };
 
     // This wave's operand rows in a stage (LDS panel rows) and exponent rows.
@@ -495,11 +543,16 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
             for (int u = 0; u < 4; ++u) acc64[s][t][u] = v4d{0.0, 0.0, 0.0, 0.0};
 
     const int nst = Qp / KS;
-    constexpr int SPB = QB / KS;  // stages per exponent block (even: buffer = stage parity)
-    static_assert(SPB % 2 == 0, "double buffer by stage parity");
+    constexpr int SPB = QB / KS;  // stages per exponent block
 #pragma unroll
-    for (int sl = 0; sl < S; ++sl) issue_slice(0, 0, sl);
-    dma_barrier<0>();
+    for (int sl = 0; sl < S; ++sl) issue_slice(0, sl);
+    if (nst > 1) {
+#pragma unroll
+        for (int sl = 0; sl < PF; ++sl) issue_slice(1, sl);
+        ring_barrier();  // stage 0 has landed
+    } else {
+        dma_barrier<0>();
+    }
     for (int qb = 0; qb < Qp / QB; ++qb) {
         // Exponents of this Q block, fetched early and without branches
         // (rows past the operand read a clamped row; those outputs are
@@ -516,22 +569,27 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 #pragma unroll
         for (int c8 = 0; c8 < SPB; ++c8) {
             const int st = qb * SPB + c8;
-            const int buf = c8 & 1;
-            const bool ahead = st + 1 < nst;
-            const char* base = lds + buf * STAGE;
+            const bool n1 = st + 1 < nst, n2 = st + 2 < nst;
             v8s bf[S][2];
 #pragma unroll
             for (int q = 0; q < S; ++q)
 #pragma unroll
                 for (int t = 0; t < 2; ++t)
-                    bf[q][t] = *reinterpret_cast<const v8s*>(base + q * SLICE + lds_off32(pcol + 32 * t + fr, h));
+                    bf[q][t] = *reinterpret_cast<const v8s*>(slot(st, q) + lds_off32(pcol + 32 * t + fr, h));
 #pragma unroll
             for (int pp = 0; pp < S; ++pp) {
-                if (ahead) issue_slice(st + 1, buf ^ 1, pp);
+                // load item pp of this stage: slice PF + pp of stage st+1,
+                // then slices 0.. of stage st+2
+                if (pp < S - PF) {
+                    if (n1) issue_slice(st + 1, PF + pp);
+                } else if (n2) {
+                    issue_slice(st + 2, pp - (S - PF));
+                }
                 v8s af[2];
+                const char* pa = slot(st, pp);
 #pragma unroll
                 for (int s = 0; s < 2; ++s)
-                    af[s] = *reinterpret_cast<const v8s*>(base + pp * SLICE + lds_off32(prow + 32 * s + fr, h));
+                    af[s] = *reinterpret_cast<const v8s*>(pa + lds_off32(prow + 32 * s + fr, h));
 #pragma unroll
                 for (int q = 0; q < S - pp; ++q)
 #pragma unroll
@@ -540,7 +598,9 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
                         for (int t = 0; t < 2; ++t)
                             acc[ozaki_tier(pp, q)][s][t] = mfma32_bf16(af[s], bf[q][t], acc[ozaki_tier(pp, q)][s][t]);
             }
-            dma_barrier<0>();
+            // stage st+1 must have landed; stage st+2's slices may stay in flight
+            if (n2) ring_barrier();
+            else dma_barrier<0>();
         }
         // FP64 flush of the QB-wide block: (tier0 + tier1) + tier2, scaled by
         // 2^(e_row + e_col - 16).
@@ -850,12 +910,13 @@ struct SliceCols {
     }
 };
 
-// Dynamic LDS of the pair kernel: two stages of every panel row and slice;
-// the symmetric kernel reuses it for the two 64 x 64 FP64 H tiles (64 KiB).
-// Dynamic LDS of the pair kernel: two stages; in the symmetric modes also the
-// epilogue exchanges (two H tiles, or three [64][65] tiles of a diagonal block).
+// Dynamic LDS of the pair kernel: the ring of slice slots (12 KiB in the
+// symmetric modes, 8 KiB in the opposite-spin mode), reused in the symmetric
+// modes by the epilogue exchanges (two H tiles, or three [64][65] tiles of a
+// diagonal block).
 constexpr size_t pair_lds(int S, bool sym) {
-    return sym ? std::max((size_t)2 * S * 12 * 1024, (size_t)3 * 64 * 65 * 8) : (size_t)2 * S * 8 * 1024;
+    const size_t ring = (size_t)ring_slots(S, sym) * (sym ? 12 : 8) * 1024;
+    return sym ? std::max(ring, (size_t)3 * 64 * 65 * 8) : ring;
 }
 
 template <int S>

@@ -480,22 +480,25 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     }
     // One slice of a stage; a stage's loads are spread over its MFMA groups
     // (a burst of them stalls instruction issue behind the address unit).
+    const int nst = Qp / KS;
     auto slot = [&](int st, int sl) { return lds + (long)((S * st + sl) % NSLOT) * SLICE; };
+    // Every wave issues exactly BPW loads per slice on every path (rows are
+    // clamped; a diagonal block's third load repeats its first, same bytes
+    // to the same place; past the last stage the source stage is clamped
+    // into slots no one reads), so the waits below count loads exactly and
+    // the compiler's own wait analysis sees one straight path.
     auto issue_slice = [&](int st, int sl) {
         char* dst = slot(st, sl);
+        const long sq = min(st, nst - 1);
 #pragma unroll
         for (int k = 0; k < BPW; ++k) {
-            if (SYM && diag && k >= 2) continue;
-            dma16(rp[k] + sl * rplane[k] + st * rstep[k], dst + rdst[k]);
+            const int kk = (SYM && diag && k >= 2) ? 0 : k;
+            dma16(rp[kk] + sl * rplane[kk] + sq * rstep[kk], dst + rdst[kk]);
         }
     };
     // Wait until at most the PF newest slices' loads of this wave are in
-    // flight (rows are clamped, so every wave of a unit issues the same
-    // number of loads per slice: BPW, or 2 in a diagonal block).
-    auto ring_barrier = [&]() {
-        if (SYM && diag) dma_barrier<PF * 2>();
-        else dma_barrier<PF * BPW>();
-    };
+    // flight, then a workgroup barrier.
+    auto ring_barrier = [&]() { dma_barrier<PF * BPW>(); };
 
     // This wave's operand rows in a stage (LDS panel rows) and exponent rows.
     int prow, pcol;          // first LDS panel row of the row / column operand
@@ -542,17 +545,12 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 #pragma unroll
             for (int u = 0; u < 4; ++u) acc64[s][t][u] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    const int nst = Qp / KS;
     constexpr int SPB = QB / KS;  // stages per exponent block
 #pragma unroll
     for (int sl = 0; sl < S; ++sl) issue_slice(0, sl);
-    if (nst > 1) {
 #pragma unroll
-        for (int sl = 0; sl < PF; ++sl) issue_slice(1, sl);
-        ring_barrier();  // stage 0 has landed
-    } else {
-        dma_barrier<0>();
-    }
+    for (int sl = 0; sl < PF; ++sl) issue_slice(1, sl);
+    ring_barrier();  // stage 0 has landed
     for (int qb = 0; qb < Qp / QB; ++qb) {
         // Exponents of this Q block, fetched early and without branches
         // (rows past the operand read a clamped row; those outputs are
@@ -569,7 +567,6 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 #pragma unroll
         for (int c8 = 0; c8 < SPB; ++c8) {
             const int st = qb * SPB + c8;
-            const bool n1 = st + 1 < nst, n2 = st + 2 < nst;
             v8s bf[S][2];
 #pragma unroll
             for (int q = 0; q < S; ++q)
@@ -580,11 +577,8 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
             for (int pp = 0; pp < S; ++pp) {
                 // load item pp of this stage: slice PF + pp of stage st+1,
                 // then slices 0.. of stage st+2
-                if (pp < S - PF) {
-                    if (n1) issue_slice(st + 1, PF + pp);
-                } else if (n2) {
-                    issue_slice(st + 2, pp - (S - PF));
-                }
+                if (pp < S - PF) issue_slice(st + 1, PF + pp);
+                else issue_slice(st + 2, pp - (S - PF));
                 v8s af[2];
                 const char* pa = slot(st, pp);
 #pragma unroll
@@ -599,8 +593,7 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
                             acc[ozaki_tier(pp, q)][s][t] = mfma32_bf16(af[s], bf[q][t], acc[ozaki_tier(pp, q)][s][t]);
             }
             // stage st+1 must have landed; stage st+2's slices may stay in flight
-            if (n2) ring_barrier();
-            else dma_barrier<0>();
+            ring_barrier();
         }
         // FP64 flush of the QB-wide block: (tier0 + tier1) + tier2, scaled by
         // 2^(e_row + e_col - 16).
@@ -617,6 +610,9 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
                     acc[0][s][t][r] = acc[1][s][t][r] = acc[2][s][t][r] = 0.f;
                 }
     }
+    // The clamped loads of the last two stages land before the LDS is
+    // reused (or handed to the next workgroup).
+    dma_barrier<0>();
 
     v4d accH[2][2][4];
     if (SYM && diag) {

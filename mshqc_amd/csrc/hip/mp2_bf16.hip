@@ -567,23 +567,32 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 #pragma unroll
         for (int c8 = 0; c8 < SPB; ++c8) {
             const int st = qb * SPB + c8;
-            v8s bf[S][2];
-#pragma unroll
-            for (int q = 0; q < S; ++q)
+            // Group pp's row fragments are read first and the column
+            // fragments in q order (group 0 uses bf[q] in that order), so
+            // the first MFMAs wait for only a few reads; the next group's row
+            // fragments are read while the current group's MFMAs run.  (The
+            // MFMA order, and so every FP32 rounding, is unchanged.)
+            auto rd_col = [&](v8s(&f)[2], int q) {
 #pragma unroll
                 for (int t = 0; t < 2; ++t)
-                    bf[q][t] = *reinterpret_cast<const v8s*>(slot(st, q) + lds_off32(pcol + 32 * t + fr, h));
+                    f[t] = *reinterpret_cast<const v8s*>(slot(st, q) + lds_off32(pcol + 32 * t + fr, h));
+            };
+            auto rd_row = [&](v8s(&f)[2], int pp) {
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    f[s] = *reinterpret_cast<const v8s*>(slot(st, pp) + lds_off32(prow + 32 * s + fr, h));
+            };
+            v8s bf[S][2], af[2], an[2];
+            rd_row(af, 0);
+#pragma unroll
+            for (int q = 0; q < S; ++q) rd_col(bf[q], q);
 #pragma unroll
             for (int pp = 0; pp < S; ++pp) {
                 // load item pp of this stage: slice PF + pp of stage st+1,
                 // then slices 0.. of stage st+2
                 if (pp < S - PF) issue_slice(st + 1, PF + pp);
                 else issue_slice(st + 2, pp - (S - PF));
-                v8s af[2];
-                const char* pa = slot(st, pp);
-#pragma unroll
-                for (int s = 0; s < 2; ++s)
-                    af[s] = *reinterpret_cast<const v8s*>(pa + lds_off32(prow + 32 * s + fr, h));
+                if (pp + 1 < S) rd_row(an, pp + 1);
 #pragma unroll
                 for (int q = 0; q < S - pp; ++q)
 #pragma unroll
@@ -591,6 +600,10 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 #pragma unroll
                         for (int t = 0; t < 2; ++t)
                             acc[ozaki_tier(pp, q)][s][t] = mfma32_bf16(af[s], bf[q][t], acc[ozaki_tier(pp, q)][s][t]);
+                if (pp + 1 < S) {
+                    af[0] = an[0];
+                    af[1] = an[1];
+                }
             }
             // stage st+1 must have landed; stage st+2's slices may stay in flight
             ring_barrier();
@@ -793,7 +806,13 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
             const bool ahead = ch + 2 < nchunk;
             const char* pa = lds + (ch % NBUF) * STAGE;
             const char* pb = pa + S * PANEL;
-            v8s b[S][4];
+            // Group p's row fragments are read first and the column fragments
+            // in q order, so the first MFMAs wait for only a few reads; the
+            // next group's row fragments are read during the current group's
+            // MFMAs (the MFMA order is unchanged).
+            v8s b[S][4], a[4], an[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) a[s] = lds_frag(pa, wm + 16 * s + rr, fs);
 #pragma unroll
             for (int q = 0; q < S; ++q)
 #pragma unroll
@@ -801,9 +820,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
 #pragma unroll
             for (int p = 0; p < S; ++p) {
                 if (ahead) issue_part((ch + 2) * KC, (ch + 2) % NBUF, p);
-                v8s a[4];
+                if (p + 1 < S) {
 #pragma unroll
-                for (int s = 0; s < 4; ++s) a[s] = lds_frag(pa + p * PANEL, wm + 16 * s + rr, fs);
+                    for (int s = 0; s < 4; ++s) an[s] = lds_frag(pa + (p + 1) * PANEL, wm + 16 * s + rr, fs);
+                }
 #pragma unroll
                 for (int q = 0; q < S - p; ++q)
 #pragma unroll
@@ -811,6 +831,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
 #pragma unroll
                         for (int t = 0; t < 4; ++t)
                             acc[p + q ? 1 : 0][s][t] = mfma_bf16(a[s], b[q][t], acc[p + q ? 1 : 0][s][t]);
+                if (p + 1 < S) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) a[s] = an[s];
+                }
             }
             // Stage ch+1 must have landed; stage ch+2 may stay in flight.
             if (ahead) dma_barrier<PER>();

@@ -88,3 +88,24 @@ def test_ump2_bf16(ctx):
     for a, b in zip(grads[:2], grads_ref[:2]):
         a = to_np(a)
         assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-6
+
+
+# UMP2 bars: the RMP2 ones, except at k = 6.  Past five slices the
+# truncation (2^-48) is below the FP32 rounding of the c >= 2 tier (near 2^-40
+# of a block sum), so k = 6 is no tighter than k = 5, and the same-spin
+# energy's antisymmetrized (ia|jb) - (ib|ja) loses a further digit to
+# cancellation (measured 4.6e-13 here, opposite spin 2.6e-14).
+UMP2_TOL = {2: 1e-4, 3: 1e-8, 4: 1e-9, 6: 1e-12}
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 6])
+def test_ump2_energy_slices(ctx, k):
+    """Every slice count of the same-spin and opposite-spin kernels (their
+    LDS rings hold a different number of slices at each k) against the
+    oracle."""
+    (Ba, eoa, eva), (Bb, eob, evb) = syn.ump2_inputs(180, 4, 150, 3, 140, 84)
+    E_ref = oracle.ump2(Ba, Bb, eoa, eva, eob, evb)
+    E = ctx.ump2_energy(*to_dev(Ba, Bb, eoa, eva, eob, evb), precision=k)
+    print(f"ump2 k={k}: {E!r} oracle {E_ref!r}", flush=True)
+    for x, y in zip(E, E_ref):
+        assert abs(x - y) <= UMP2_TOL[k] * abs(y)

@@ -26,8 +26,9 @@ struct Bf16Operands {
     // Gradient-GEMM operands, per occupied block of bo orbitals: block b
     // holds columns (i, a) of its orbitals as one compact run of nb*v values,
     // padded once to kw = round_up(nb*v, 256): slices [S][Q][kw] at
-    // g + b * S*Q*kwb and exponents [Q][kw/256] at ge + b * Q*kwb/256, with
-    // kwb the kw of a full block.  Left and right.
+    // g + b * S*Q*kwb and exponents [kw/256][Qx] (Qx = Q rounded up to 128)
+    // at ge + b * Qx*kwb/256, with kwb the kw of a full block.  Left and
+    // right.
     int bol = 0, bor = 0;
     long kwbl = 0, kwbr = 0;
     const uint16_t* gl = nullptr;

@@ -37,6 +37,7 @@ constexpr int LROW = 64;  // LDS bytes per 32-element BF16 row
 // well inside its bar.
 constexpr int GB = 256;
 constexpr int GT = 128;   // gradient-GEMM workgroup tile (rows and columns)
+constexpr int GW = 8;     // gradient-GEMM waves per workgroup (two per SIMD, 64 x 32 each)
 
 // Slots of the pair kernel's LDS ring of slices (12 KiB per slice in the
 // symmetric modes, 8 KiB in the opposite-spin mode) within 160 KiB: two whole
@@ -125,13 +126,15 @@ __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restric
 }
 
 // src[R][ld] with column index seg*v + b (seg < nseg, b < v)  ->
-// dst[S][R][nseg*vp] (column seg*vp + b, zero padded), ex[R][nseg*(vp/GB)].
-// One wave per (row, seg, GB-wide b tile); each lane slices GB/64 elements.
+// dst[S][R][nseg*vp] (column seg*vp + b, zero padded), exponents
+// ex[nseg*(vp/GB)][ldx] (k-block major, ldx >= R rows: four consecutive rows
+// are one dword).  One wave per (row, seg, GB-wide b tile); each lane slices
+// GB/64 elements.
 template <int S>
 __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restrict__ src, long ld, int R,
                                                          int nseg, int v, int vp,
                                                          uint16_t* __restrict__ dst,
-                                                         int8_t* __restrict__ ex) {
+                                                         int8_t* __restrict__ ex, long ldx) {
     const int ntv = vp / GB;
     const long job = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const long njobs = (long)R * nseg * ntv;
@@ -162,20 +165,21 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
 #pragma unroll
         for (int p = 0; p < S; ++p) d[p * slice + 64 * u] = sv[p];
     }
-    if (lane == 0) ex[(long)row * nseg * ntv + rem] = (int8_t)e;
+    if (lane == 0) ex[(long)rem * ldx + row] = (int8_t)e;
 }
 
 // Transposed slicing of Z[(i',a)][(j',b)] (rows i'*v + a, i' < nI; ld = ldz;
 // nc = nJ*vr columns):  dst[S][nc][nI*vp] with row n = column of Z and
-// column i'*vp + a;  ex[nc][nI*(vp/GB)].  A workgroup transposes a tile of
-// SC_COLS columns x GB values of a through LDS.
+// column i'*vp + a;  exponents ex[nI*(vp/GB)][ldx] (k-block major, ldx >= nc).
+// A workgroup transposes a tile of SC_COLS columns x GB values of a through
+// LDS.
 constexpr int SC_COLS = 16;
 
 template <int S>
 __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restrict__ Z, long ldz, int nI,
                                                          int v, int vp, int nc,
                                                          uint16_t* __restrict__ dst,
-                                                         int8_t* __restrict__ ex) {
+                                                         int8_t* __restrict__ ex, long ldx) {
     constexpr int NC = SC_COLS, PARTS = 256 / NC, PER = GB / PARTS;  // PER values of a per thread
     static_assert(PER % 8 == 0, "whole 16-byte stores");
     __shared__ double T[GB][NC + 1];
@@ -221,7 +225,7 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
         for (int p = 0; p < S; ++p)
             *reinterpret_cast<uint4*>(d + p * slice + u8) = *reinterpret_cast<uint4*>(out[p]);
     }
-    if (part == 0) ex[(long)(c0 + cc) * nI * ntv + ip * ntv + at] = (int8_t)e;
+    if (part == 0) ex[(long)(ip * ntv + at) * ldx + c0 + cc] = (int8_t)e;
 }
 
 // ------------------------------------------------ shared MFMA helpers
@@ -714,144 +718,135 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 struct SlicedMat {
     const uint16_t* s;  // [S] planes, plane stride `plane`
     long plane, ld;     // elements
-    const int8_t* e;    // exponents [row][kb], row stride `lde`
+    const int8_t* e;    // exponents [kb][row], k-block stride `lde` (a multiple of GT >= rows)
     long lde;
     int rows;
 };
 
 // C[m][n] += sum_k A[m][k] X[n][k] for the gradient contractions; K is a
-// multiple of GB.  128 x 128 workgroup tile, four waves of 64 x 64, one
-// 32-wide k step per stage; stages are loaded straight into a triple-buffered
-// LDS by the DMA path, two stages ahead (one barrier per stage, no staging
-// registers).  The
+// multiple of GB.  128 x 128 workgroup tile, eight waves of 64 x 32 (two
+// per SIMD, so one wave's MFMAs fill the other's load, LDS and barrier
+// gaps), one 32-wide k step per stage; stages are loaded straight into a
+// triple-buffered LDS by the DMA path, two stages ahead (one barrier per
+// stage, no staging registers).  The
 // gradient needs ~1e-7 relative accuracy, so products go to two FP32 tiers:
 // c = 0 (integers, exact up to the last bit above 2^23) and c >= 1 (rounding
 // near 2^-31 of the block scale); both are flushed to FP64 with the block
 // exponents once per GB-wide k block.
 template <int S>
-__global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMat X, int K, double* __restrict__ C,
-                                                           long ldc) {
+__global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMat X, int K,
+                                                               double* __restrict__ C, long ldc) {
     constexpr long PANEL = (long)GT * LROW;   // bytes of one slice panel (128 rows x 32 k)
     constexpr long STAGE = 2L * S * PANEL;    // both operands, all slices
     constexpr int NBUF = 3;
-    constexpr int PER = 2 * S * (GT / 16) / 4; // 16-row DMA loads per wave and stage
+    constexpr int PER = 2 * S * (GT / 16) / GW; // 16-row DMA loads per wave and stage
+    static_assert(PER == 2 * S, "each wave loads one 16-row block of every operand slice");
     extern __shared__ __align__(16) char lds[];
     int mt, ntile;
     if (!grouped_tile((A.rows + GT - 1) / GT, (X.rows + GT - 1) / GT, mt, ntile)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    // eight waves, two per SIMD: wave w owns rows wm .. wm+63, columns
+    // wn .. wn+31 of the 128 x 128 tile
+    const int wm = (wave >> 2) * 64, wn = (wave & 3) * 32;
     const int m0 = mt * GT, n0 = ntile * GT;
-    // DMA load u of this wave covers operand P, slice sl and panel rows
-    // 16*rb .. 16*rb+15 (id = wave + 4u enumerates (P, sl, rb)).  Rows past
-    // the matrix are clamped to its last row (those LDS rows only feed
-    // discarded outputs): every wave then issues exactly PER loads per
-    // stage, which the vmcnt(PER) stage barrier relies on.  (A load skipped
-    // for an all-out-of-range block would let the barrier pass before the
-    // previous stage had landed.)
+    // DMA load u of wave w covers operand P = u / S, slice sl = u % S and
+    // panel rows 16w .. 16w+15.  Rows past the matrix are clamped to its last
+    // row, and chunks past the end to the last chunk (those LDS bytes only
+    // feed discarded outputs or are never read): every wave issues exactly
+    // PER loads per chunk on every path, which the vmcnt(PER) chunk barrier
+    // relies on.
     const int wv = __builtin_amdgcn_readfirstlane(wave);
     const int lrow = lane >> 2;
     const int lseg8 = 8 * lane_seg(lane);
-    // Loads u in [4g, 4g+4) of a stage; a stage's loads are spread over its
-    // S MFMA groups (PER = 4S).
-    auto issue_part = [&](int k0, int buf, int g) {
+    const int nkb = K / GB;
+    constexpr int CPB = GB / KC;  // chunks per exponent block
+    const int nchunk = nkb * CPB;
+    // Loads u in [2g, 2g+2) of a chunk; a chunk's loads are spread over its
+    // S MFMA groups.
+    auto issue_part = [&](int ch, int g) {
+        const int k0 = min(ch, nchunk - 1) * KC;
+        char* dst = lds + (ch % NBUF) * STAGE;
 #pragma unroll
-        for (int u = 4 * g; u < 4 * g + 4; ++u) {
-            const int id = wv + 4 * u;
-            const int P = id / (8 * S), sl = (id / 8) % S, rb = id % 8;
-            const SlicedMat& M = P ? X : A;
-            const int r = min((P ? n0 : m0) + 16 * rb + lrow, M.rows - 1);
-            dma16(M.s + sl * M.plane + (long)r * M.ld + k0 + lseg8,
-                  lds + buf * STAGE + (P * S + sl) * PANEL + 1024 * rb);
+        for (int u = 2 * g; u < 2 * g + 2; ++u) {
+            const int Pn = u / S, sl = u % S;
+            const SlicedMat& M = Pn ? X : A;
+            const int r = min((Pn ? n0 : m0) + 16 * wv + lrow, M.rows - 1);
+            dma16(M.s + sl * M.plane + (long)r * M.ld + k0 + lseg8, dst + (Pn * S + sl) * PANEL + 1024 * wv);
         }
     };
-    v4f acc[2][4][4];
-    v4d acc64[4][4];
+    v4f acc[2][4][2];
+    v4d acc64[4][2];
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < 2; ++t) {
             acc64[s][t] = v4d{0.0, 0.0, 0.0, 0.0};
             acc[0][s][t] = acc[1][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
         }
     const int rr = lane & 15, fs = lane >> 4;
-    const int nkb = K / GB;
-    constexpr int CPB = GB / KC;  // chunks per exponent block (even: buffer = chunk parity)
-    const int nchunk = nkb * CPB;  // >= 4
-    static_assert(PER == 4 * S, "four loads per MFMA group");
 #pragma unroll
-    for (int g = 0; g < S; ++g) issue_part(0, 0, g);
+    for (int g = 0; g < S; ++g) issue_part(0, g);
 #pragma unroll
-    for (int g = 0; g < S; ++g) issue_part(KC, 1, g);
+    for (int g = 0; g < S; ++g) issue_part(1, g);
     dma_barrier<PER>();
     for (int kb = 0; kb < nkb; ++kb) {
         // Exponents of this block, fetched early and without branches (rows
         // past the matrix read the last row's: their outputs are discarded),
         // so the loads issue back to back and their wait lands at the flush.
-        int eA[4][4], eB[4];
+        // The tables are padded to whole 128-row tiles, so no row is clamped;
+        // the four rows 4fs .. 4fs+3 of a 16-row block are one dword.
+        int eA[4], eB[2];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int n = min(n0 + wn + 16 * t + rr, X.rows - 1);
-            eB[t] = X.e[(long)n * X.lde + kb];
-        }
+        for (int t = 0; t < 2; ++t) eB[t] = X.e[(long)kb * X.lde + n0 + wn + 16 * t + rr];
 #pragma unroll
         for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = min(m0 + wm + 16 * s + 4 * fs + r, A.rows - 1);
-                eA[s][r] = A.e[(long)m * A.lde + kb];
-            }
+            eA[s] = *reinterpret_cast<const int*>(A.e + (long)kb * A.lde + m0 + wm + 16 * s + 4 * fs);
 #pragma unroll
         for (int c = 0; c < CPB; ++c) {
             const int ch = kb * CPB + c;
-            const bool ahead = ch + 2 < nchunk;
             const char* pa = lds + (ch % NBUF) * STAGE;
             const char* pb = pa + S * PANEL;
-            // Group p's row fragments are read first and the column fragments
-            // in q order, so the first MFMAs wait for only a few reads; the
-            // next group's row fragments are read during the current group's
-            // MFMAs (the MFMA order is unchanged).
-            v8s b[S][4], a[4], an[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) a[s] = lds_frag(pa, wm + 16 * s + rr, fs);
+            // Row fragments first, column fragments in q order (group 0 uses
+            // them in that order); the other wave on the SIMD covers the
+            // read latency of each group's row fragments.
+            v8s b[S][2];
 #pragma unroll
             for (int q = 0; q < S; ++q)
 #pragma unroll
-                for (int t = 0; t < 4; ++t) b[q][t] = lds_frag(pb + q * PANEL, wn + 16 * t + rr, fs);
+                for (int t = 0; t < 2; ++t) b[q][t] = lds_frag(pb + q * PANEL, wn + 16 * t + rr, fs);
 #pragma unroll
             for (int p = 0; p < S; ++p) {
-                if (ahead) issue_part((ch + 2) * KC, (ch + 2) % NBUF, p);
-                if (p + 1 < S) {
+                issue_part(ch + 2, p);
+                v8s a[4];
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) an[s] = lds_frag(pa + (p + 1) * PANEL, wm + 16 * s + rr, fs);
-                }
+                for (int s = 0; s < 4; ++s) a[s] = lds_frag(pa + p * PANEL, wm + 16 * s + rr, fs);
 #pragma unroll
                 for (int q = 0; q < S - p; ++q)
 #pragma unroll
                     for (int s = 0; s < 4; ++s)
 #pragma unroll
-                        for (int t = 0; t < 4; ++t)
+                        for (int t = 0; t < 2; ++t)
                             acc[p + q ? 1 : 0][s][t] = mfma_bf16(a[s], b[q][t], acc[p + q ? 1 : 0][s][t]);
-                if (p + 1 < S) {
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) a[s] = an[s];
-                }
             }
-            // Stage ch+1 must have landed; stage ch+2 may stay in flight.
-            if (ahead) dma_barrier<PER>();
-            else dma_barrier<0>();
+            // Chunk ch+1 must have landed; chunk ch+2 may stay in flight.
+            dma_barrier<PER>();
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < 2; ++t) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
+                    const int er = (int)(signed char)(eA[s] >> (8 * r));
                     const double sum = (double)acc[0][s][t][r] + (double)acc[1][s][t][r];
-                    acc64[s][t][r] += sum * pow2(eA[s][r] + eB[t] - 16);
+                    acc64[s][t][r] += sum * pow2(er + eB[t] - 16);
                 }
                 acc[0][s][t] = acc[1][s][t] = v4f{0.f, 0.f, 0.f, 0.f};
             }
     }
+    // The clamped loads of the last two chunks land before the LDS is
+    // handed to the next workgroup.
+    dma_barrier<0>();
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -859,7 +854,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMa
             const int m = m0 + wm + 16 * s + 4 * fs + r;
             if (m >= A.rows) continue;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < 2; ++t) {
                 const int n = n0 + wn + 16 * t + rr;
                 if (n < X.rows) C[(long)m * ldc + n] += acc64[s][t][r];
             }
@@ -911,9 +906,9 @@ struct SlicePair {
 template <int S>
 struct SliceRows {
     static int run(mshqc_hip_ctx* ctx, const double* src, long ld, int R, int nseg, int v, int vp,
-                   uint16_t* dst, int8_t* ex) {
+                   uint16_t* dst, int8_t* ex, long ldx) {
         const long jobs = (long)R * nseg * (vp / GB);
-        slice_rows_kernel<S><<<ceil_div(jobs, 4), 256, 0, ctx->stream>>>(src, ld, R, nseg, v, vp, dst, ex);
+        slice_rows_kernel<S><<<ceil_div(jobs, 4), 256, 0, ctx->stream>>>(src, ld, R, nseg, v, vp, dst, ex, ldx);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -922,9 +917,9 @@ struct SliceRows {
 template <int S>
 struct SliceCols {
     static int run(mshqc_hip_ctx* ctx, const double* Z, long ldz, int nI, int v, int vp, int nc,
-                   uint16_t* dst, int8_t* ex) {
+                   uint16_t* dst, int8_t* ex, long ldx) {
         dim3 grid(ceil_div(nc, SC_COLS), nI * (vp / GB));
-        slice_cols_kernel<S><<<grid, 256, 0, ctx->stream>>>(Z, ldz, nI, v, vp, nc, dst, ex);
+        slice_cols_kernel<S><<<grid, 256, 0, ctx->stream>>>(Z, ldz, nI, v, vp, nc, dst, ex, ldx);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -975,7 +970,7 @@ struct GemmLaunch {
         if (K % GB != 0)
             return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 GEMM: K=%d is not a multiple of %d", K, GB);
         dim3 grid(grouped_grid(ceil_div(A.rows, GT), ceil_div(X.rows, GT)));
-        gemm_bf16_nt_kernel<S><<<grid, 256, gemm_lds(S), ctx->stream>>>(A, X, K, C, ldc);
+        gemm_bf16_nt_kernel<S><<<grid, GW * 64, gemm_lds(S), ctx->stream>>>(A, X, K, C, ldc);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -1055,7 +1050,8 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
         o.kwbr = Br ? round_up(o.bor * vr, GB) : o.kwbl;
         const int nbl = ceil_div(ol, o.bol), nbr = Br ? ceil_div(orr, o.bor) : 0;
         const size_t gl = (size_t)nbl * SG * Q * o.kwbl * 2, gr = (size_t)nbr * SG * Q * o.kwbr * 2;
-        const size_t gel = (size_t)nbl * Q * (o.kwbl / GB), ger = (size_t)nbr * Q * (o.kwbr / GB);
+        const long Qx = round_up(Q, GT);  // exponent tables: [kb][Qx] per block
+        const size_t gel = (size_t)nbl * Qx * (o.kwbl / GB), ger = (size_t)nbr * Qx * (o.kwbr / GB);
         char* g = static_cast<char*>(workspace(ctx, WS_SLICE2, gl + gr + gel + ger + 1024));
         if (!g) return MSHQC_HIP_ERR_ALLOC;
         o.gl = reinterpret_cast<uint16_t*>(g);
@@ -1072,7 +1068,7 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
                 const int nb = std::min(bo, oo - i0);
                 rc = by_gslices<SliceRows>(SG, ctx, src + (long)i0 * vv, (long)oo * vv, Q, 1, nb * vv,
                                           round_up(nb * vv, GB), dst + (size_t)b * SG * Q * kwb,
-                                          ex + (size_t)b * Q * (kwb / GB));
+                                          ex + (size_t)b * Qx * (kwb / GB), Qx);
                 if (rc) return rc;
             }
         }
@@ -1100,35 +1096,35 @@ int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& o, const double* 
     int rc;
     if (dBl) {
         // dBl[q][(i,a)] += sum_{(j,b)} Br[q][(j,b)] Z[(i,a)][(j,b)]
-        const long R = (long)nI * vl, kw = round_up(nJ * vr, GB);
-        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * (kw / GB);
+        const long R = (long)nI * vl, kw = round_up(nJ * vr, GB), Rx = round_up(R, GT), Qx = round_up(Q, GT);
+        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)Rx * (kw / GB);
         char* w = static_cast<char*>(workspace(ctx, WS_ZS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;
         uint16_t* zs = reinterpret_cast<uint16_t*>(w);
         int8_t* zx = reinterpret_cast<int8_t*>(w + zb);
-        rc = by_gslices<SliceRows>(S, ctx, Z, ldz, (int)R, 1, nJ * vr, (int)kw, zs, zx);
+        rc = by_gslices<SliceRows>(S, ctx, Z, ldz, (int)R, 1, nJ * vr, (int)kw, zs, zx, Rx);
         if (rc) return rc;
         const long b = J0 / o.bor;
-        SlicedMat A{o.gr + (size_t)b * S * Q * o.kwbr, (long)Q * kw, kw, o.ger + (size_t)b * Q * (o.kwbr / GB),
-                    kw / GB, Q};
-        SlicedMat X{zs, (long)R * kw, kw, zx, kw / GB, (int)R};
+        SlicedMat A{o.gr + (size_t)b * S * Q * o.kwbr, (long)Q * kw, kw, o.ger + (size_t)b * Qx * (o.kwbr / GB),
+                    Qx, Q};
+        SlicedMat X{zs, (long)R * kw, kw, zx, Rx, (int)R};
         rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBl + (long)I0 * vl, (long)o.ol * vl);
         if (rc) return rc;
     }
     if (second && dBr) {
         // dBr[q][(j,b)] += sum_{(i,a)} Bl[q][(i,a)] Z[(i,a)][(j,b)]
-        const long R = (long)nJ * vr, kw = round_up(nI * vl, GB);
-        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)R * (kw / GB);
+        const long R = (long)nJ * vr, kw = round_up(nI * vl, GB), Rx = round_up(R, GT), Qx = round_up(Q, GT);
+        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)Rx * (kw / GB);
         char* w = static_cast<char*>(workspace(ctx, WS_ZTS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;
         uint16_t* zs = reinterpret_cast<uint16_t*>(w);
         int8_t* zx = reinterpret_cast<int8_t*>(w + zb);
-        rc = by_gslices<SliceCols>(S, ctx, Z, ldz, 1, nI * vl, (int)kw, (int)R, zs, zx);
+        rc = by_gslices<SliceCols>(S, ctx, Z, ldz, 1, nI * vl, (int)kw, (int)R, zs, zx, Rx);
         if (rc) return rc;
         const long b = I0 / o.bol;
-        SlicedMat A{o.gl + (size_t)b * S * Q * o.kwbl, (long)Q * kw, kw, o.gel + (size_t)b * Q * (o.kwbl / GB),
-                    kw / GB, Q};
-        SlicedMat X{zs, (long)R * kw, kw, zx, kw / GB, (int)R};
+        SlicedMat A{o.gl + (size_t)b * S * Q * o.kwbl, (long)Q * kw, kw, o.gel + (size_t)b * Qx * (o.kwbl / GB),
+                    Qx, Q};
+        SlicedMat X{zs, (long)R * kw, kw, zx, Rx, (int)R};
         rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBr + (long)J0 * vr, (long)o.orr * vr);
         if (rc) return rc;
     }

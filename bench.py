@@ -102,20 +102,36 @@ def cpu_baseline(B, eo, ev, npairs, seed=7):
 
 
 def timed(torch, ctx, fn, steps, warmup):
-    """Seconds per step of fn() and the per-kernel-class profile of the timed steps."""
+    """Seconds per step of fn() (wall clock over all steps, synchronized on
+    both sides) and the per-kernel-class profile of the timed steps; the
+    profile's "step_ms" holds each step's device time from HIP events on the
+    stream the library runs on (it follows torch's current stream)."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     ctx.profile(True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
     out = None
-    for _ in range(steps):
+    for a, b in evs:
+        a.record()
         out = fn()
+        b.record()
     torch.cuda.synchronize()
     sec = (time.perf_counter() - t0) / steps
     prof = {k: ctx.profile_get(k) for k in (0, 1, 2)}
+    prof["step_ms"] = [a.elapsed_time(b) for a, b in evs]
     ctx.profile(False)
     return sec, out, prof
+
+
+def step_stats(step_ms):
+    """Median, min and max of the per-step device times, in seconds."""
+    x = sorted(step_ms)
+    n = len(x)
+    med = x[n // 2] if n % 2 else 0.5 * (x[n // 2 - 1] + x[n // 2])
+    return {"median": med * 1e-3, "min": x[0] * 1e-3, "max": x[-1] * 1e-3, "n": n,
+            "timer": "HIP events around each step on the library's stream"}
 
 
 def calibrate(torch, mib):
@@ -129,8 +145,8 @@ def calibrate(torch, mib):
     log(f"[bench] calibration sqrt: {nel * 8} bytes read, {nel * 8} bytes written")
 
 
-def base_line(args, sec, dtype, config, flop):
-    return {
+def base_line(args, sec, dtype, config, flop, step_ms=None):
+    line = {
         "metric": METRIC,
         "value": sec,
         "unit": "s",
@@ -146,6 +162,9 @@ def base_line(args, sec, dtype, config, flop):
         "config": config,
         "tflops": flop / sec / 1e12 if flop else None,
     }
+    if step_ms:
+        line["step_seconds"] = step_stats(step_ms)
+    return line
 
 
 def mp2_line(args, results, precs, W3, config, energy_of, grads_of):
@@ -162,7 +181,7 @@ def mp2_line(args, results, precs, W3, config, energy_of, grads_of):
     peak = FP64_PEAK_TFLOPS if main_p == 0 else BF16_PEAK_TFLOPS
     pair_tflops = W / (pair_ms_step * 1e-3) / 1e12 if pair_ms_step > 0 else 0.0
     hw_factor = 1 if main_p == 0 else main_p * (main_p + 1) // 2
-    line = base_line(args, r["sec"], prec_name(main_p), config, W3)
+    line = base_line(args, r["sec"], prec_name(main_p), config, W3, r["prof"]["step_ms"])
     line["energy"] = energy_of(r["out"])
     line["kernels"] = {
         "pair_kernel": {"launches_per_step": npair_launch / steps, "ms_per_step": pair_ms_step,
@@ -285,7 +304,7 @@ def run_jk(args, torch, hip, syn, w):
         sec, _, prof = timed(torch, ctx, fn, args.steps, args.warmup)
         j_ms = prof[2][1] / args.steps
         out[name] = {"seconds": sec, "flop": flops, "tflops": flops / sec / 1e12,
-                     "j_kernel_ms": j_ms if j_ms > 0 else None}
+                     "j_kernel_ms": j_ms if j_ms > 0 else None, "step_seconds": step_stats(prof["step_ms"])}
         total += sec
         log(f"[bench] {name}: {sec * 1e3:.2f} ms/step, {flops / sec / 1e12:.1f} TFLOP/s, J kernels {prof[2]}")
     jb = 16.0 * Q * n * n  # two passes over L
@@ -293,6 +312,7 @@ def run_jk(args, torch, hip, syn, w):
     config = {"workload": f"{args.workload}: CD-RHF/ROHF Fock pieces from L", "Q": Q, "n": n, "o": o,
               "parallelism": "single GPU"}
     line = base_line(args, total, "f64", config, sum(f for _, f in parts.values()))
+    line["step_seconds"] = {name: v["step_seconds"] for name, v in out.items()}
     line["kernels"] = out
     if j_ms:
         achieved = jb / (j_ms * 1e-3) / 1e9
@@ -332,7 +352,7 @@ def run_omp2(args, torch, hip, syn, w):
         f"E = {E}, pair GEMMs {prof[0]}, OPDM GEMMs {prof[1]}")
     config = {"workload": f"{args.workload}: CD-OMP2 amplitudes, energy and OPDM per iteration", "Q": Q,
               "oa": oa, "va": va, "ob": ob, "vb": vb, "parallelism": "single GPU"}
-    line = base_line(args, sec, "f64", config, flop)
+    line = base_line(args, sec, "f64", config, flop, prof["step_ms"])
     achieved = flop / sec / 1e12
     line["roofline"] = {"kernel": "FP64 MFMA GEMMs (pair integrals + OPDM contractions)", "bound": "mfma",
                         "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s", "frac": achieved / 78.6,
@@ -351,7 +371,7 @@ def run_cd(args, torch, hip, syn, w):
     Ls = torch.from_numpy(syn.workload(args.workload)).cuda()
     M = (Ls.t() @ Ls).contiguous()
     del Ls
-    sec, out, _ = timed(torch, ctx, lambda: ctx.cholesky(M, thr), args.steps, args.warmup)
+    sec, out, prof = timed(torch, ctx, lambda: ctx.cholesky(M, thr), args.steps, args.warmup)
     L, conv, res = out
     q = L.shape[0]
     # HBM bytes the algorithm moves: step k reads k earlier vectors and the
@@ -362,7 +382,7 @@ def run_cd(args, torch, hip, syn, w):
         f"{achieved:.0f} GB/s algorithmic")
     config = {"workload": f"{args.workload}: pivoted Cholesky of an n^2 x n^2 PSD matrix", "n": n,
               "rank": Q0, "threshold": thr, "vectors": q, "parallelism": "single GPU"}
-    line = base_line(args, sec, "f64", config, 0.0)
+    line = base_line(args, sec, "f64", config, 0.0, prof["step_ms"])
     line["roofline"] = {"kernel": "pivot scan + new vector (reads the earlier vectors)", "bound": "hbm",
                         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": None}
@@ -398,7 +418,7 @@ def run_rmp3(args, torch, hip, syn, w):
         f"GEMMs {prof[1]}")
     config = {"workload": f"{args.workload}: CD-RMP3 second-order amplitudes and energy", "Q": Q, "n": n,
               "o": o, "v": v, "parallelism": "single GPU"}
-    line = base_line(args, sec, "f64", config, flop)
+    line = base_line(args, sec, "f64", config, flop, prof["step_ms"])
     achieved = flop / sec / 1e12
     line["roofline"] = {"kernel": "FP64 MFMA GEMMs ((ac|bd) build and ladder contractions)", "bound": "mfma",
                         "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s", "frac": achieved / 78.6,
@@ -447,7 +467,7 @@ def run_ump3(args, torch, hip, syn, w):
         f"GEMMs {prof[1]}")
     config = {"workload": f"{args.workload}: CD-UMP3 energy components", "Q": Q, "n": n, "oa": oa, "va": va,
               "ob": ob, "vb": vb, "parallelism": "single GPU"}
-    line = base_line(args, sec, "f64", config, flop)
+    line = base_line(args, sec, "f64", config, flop, prof["step_ms"])
     achieved = flop / sec / 1e12
     line["roofline"] = {"kernel": "FP64 MFMA GEMMs ((ac|bd) build, ladders, rings)", "bound": "mfma",
                         "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s", "frac": achieved / 78.6,
@@ -459,7 +479,7 @@ def run_ump3(args, torch, hip, syn, w):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16x5", help="fp64 | bf16xK (K slices)")
     ap.add_argument("--compare", default="fp64", help="second precision timed beside it ('' for none)")

@@ -495,8 +495,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are out of "
-                                   "scope (see DESIGN.md for how the path would shard)",
+        print(json.dumps({"error": "one GPU only: the measured value is one GPU's throughput, and "
+                                   "runs on more than one GPU are out of scope",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

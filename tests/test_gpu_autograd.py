@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def cfg3():
+def o40_inputs():
     import torch
 
     B, eo, ev = syn.workload("mp2_o40")
@@ -20,27 +20,27 @@ def cfg3():
 
 
 @pytest.mark.parametrize("precision", [0, 5])
-def test_fd_64_entries(ctx, cfg3, precision):
+def test_fd_64_entries(ctx, o40_inputs, precision):
     import torch
 
     from mshqc_amd.autograd import rmp2_energy
 
-    B, eo, ev = [t.clone().requires_grad_(True) for t in cfg3]
+    B, eo, ev = [t.clone().requires_grad_(True) for t in o40_inputs]
     E = rmp2_energy(B, eo, ev, precision)
     E.backward()
     g = B.grad
     rng = np.random.default_rng(64)
     Q, o, v = B.shape
     h = 1e-5
-    Bd = cfg3[0].clone()
+    Bd = o40_inputs[0].clone()
     fds, gs = [], []
     for _ in range(64):
         idx = (int(rng.integers(Q)), int(rng.integers(o)), int(rng.integers(v)))
         x0 = Bd[idx].item()
         Bd[idx] = x0 + h
-        Ep = ctx.rmp2_energy(Bd, cfg3[1], cfg3[2])
+        Ep = ctx.rmp2_energy(Bd, o40_inputs[1], o40_inputs[2])
         Bd[idx] = x0 - h
-        Em = ctx.rmp2_energy(Bd, cfg3[1], cfg3[2])
+        Em = ctx.rmp2_energy(Bd, o40_inputs[1], o40_inputs[2])
         Bd[idx] = x0
         fds.append((Ep - Em) / (2 * h))
         gs.append(g[idx].item())
@@ -55,17 +55,17 @@ def test_fd_64_entries(ctx, cfg3, precision):
     assert rel < 1e-6
     # orbital energies
     for k in (0, 17, 39):
-        e0 = cfg3[1].clone()
+        e0 = o40_inputs[1].clone()
         e0[k] += h
-        Ep = ctx.rmp2_energy(cfg3[0], e0, cfg3[2])
+        Ep = ctx.rmp2_energy(o40_inputs[0], e0, o40_inputs[2])
         e0[k] -= 2 * h
-        Em = ctx.rmp2_energy(cfg3[0], e0, cfg3[2])
+        Em = ctx.rmp2_energy(o40_inputs[0], e0, o40_inputs[2])
         fd = (Ep - Em) / (2 * h)
         assert abs(fd - eo.grad[k].item()) <= 1e-5 * abs(fd)
 
 
-def test_identities_at_full_size(ctx, cfg3):
-    B, eo, ev = cfg3
+def test_identities_at_full_size(ctx, o40_inputs):
+    B, eo, ev = o40_inputs
     E, g, d_o, d_v = ctx.rmp2_grad(B, eo, ev)
     assert abs((B * g).sum().item() - 4 * E) <= 1e-12 * abs(E)
     assert abs(d_o.sum().item() + d_v.sum().item()) <= 1e-12 * abs(E)

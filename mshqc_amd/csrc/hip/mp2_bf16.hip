@@ -485,14 +485,35 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     // One slice of a stage; a stage's loads are spread over its MFMA groups
     // (a burst of them stalls instruction issue behind the address unit).
     const int nst = Qp / KS;
-    auto slot = [&](int st, int sl) { return lds + (long)((S * st + sl) % NSLOT) * SLICE; };
+    // Ring state, as LDS byte offsets: cur[sl] holds slice sl of the current
+    // stage st, lo[k] slice k < PF of stage st+1 (loaded during stage st-1),
+    // fslot[g] the slots stage st-1 read, which stage st's load items g fill
+    // (items g < S-PF: slice PF+g of stage st+1; the rest: slice g-(S-PF) of
+    // stage st+2).  Each stage permutes these 2S+PF offsets; inside the
+    // unrolled stages that is register renaming, where the modulo form costs
+    // ~60 scalar instructions a stage.  The symmetric S = 6 kernel has no
+    // registers to spare for the state and keeps the modulo form.
+    constexpr bool RREG = !(SYM && S >= 6);
+    int cur[S], lo[PF > 0 ? PF : 1], fslot[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        cur[k] = k * (int)SLICE;
+        fslot[k] = (S + PF + k) * (int)SLICE;
+    }
+#pragma unroll
+    for (int k = 0; k < PF; ++k) lo[k] = (S + k) * (int)SLICE;
+    auto mslot = [&](int st, int sl) { return (int)(((S * st + sl) % NSLOT) * SLICE); };
+    // offset of slice q of stage st, and the target of load item g in stage st
+    auto rd_off = [&](int st, int q) { return RREG ? cur[q] : mslot(st, q); };
+    auto ld_off = [&](int st, int g) {
+        return RREG ? fslot[g] : g < S - PF ? mslot(st + 1, PF + g) : mslot(st + 2, g - (S - PF));
+    };
     // Every wave issues exactly BPW loads per slice on every path (rows are
     // clamped; a diagonal block's third load repeats its first, same bytes
     // to the same place; past the last stage the source stage is clamped
     // into slots no one reads), so the waits below count loads exactly and
     // the compiler's own wait analysis sees one straight path.
-    auto issue_slice = [&](int st, int sl) {
-        char* dst = slot(st, sl);
+    auto issue_slice = [&](int st, int sl, char* dst) {
         const long sq = min(st, nst - 1);
 #pragma unroll
         for (int k = 0; k < BPW; ++k) {
@@ -551,9 +572,9 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 
     constexpr int SPB = QB / KS;  // stages per exponent block
 #pragma unroll
-    for (int sl = 0; sl < S; ++sl) issue_slice(0, sl);
+    for (int sl = 0; sl < S; ++sl) issue_slice(0, sl, lds + cur[sl]);
 #pragma unroll
-    for (int sl = 0; sl < PF; ++sl) issue_slice(1, sl);
+    for (int sl = 0; sl < PF; ++sl) issue_slice(1, sl, lds + lo[sl]);
     ring_barrier();  // stage 0 has landed
     for (int qb = 0; qb < Qp / QB; ++qb) {
         // Exponents of this Q block, fetched early and without branches
@@ -579,12 +600,12 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
             auto rd_col = [&](v8s(&f)[2], int q) {
 #pragma unroll
                 for (int t = 0; t < 2; ++t)
-                    f[t] = *reinterpret_cast<const v8s*>(slot(st, q) + lds_off32(pcol + 32 * t + fr, h));
+                    f[t] = *reinterpret_cast<const v8s*>(lds + rd_off(st, q) + lds_off32(pcol + 32 * t + fr, h));
             };
             auto rd_row = [&](v8s(&f)[2], int pp) {
 #pragma unroll
                 for (int s = 0; s < 2; ++s)
-                    f[s] = *reinterpret_cast<const v8s*>(slot(st, pp) + lds_off32(prow + 32 * s + fr, h));
+                    f[s] = *reinterpret_cast<const v8s*>(lds + rd_off(st, pp) + lds_off32(prow + 32 * s + fr, h));
             };
             v8s bf[S][2], af[2], an[2];
             rd_row(af, 0);
@@ -594,8 +615,8 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
             for (int pp = 0; pp < S; ++pp) {
                 // load item pp of this stage: slice PF + pp of stage st+1,
                 // then slices 0.. of stage st+2
-                if (pp < S - PF) issue_slice(st + 1, PF + pp);
-                else issue_slice(st + 2, pp - (S - PF));
+                if (pp < S - PF) issue_slice(st + 1, PF + pp, lds + ld_off(st, pp));
+                else issue_slice(st + 2, pp - (S - PF), lds + ld_off(st, pp));
                 if (pp + 1 < S) rd_row(an, pp + 1);
 #pragma unroll
                 for (int q = 0; q < S - pp; ++q)
@@ -611,6 +632,23 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
             }
             // stage st+1 must have landed; stage st+2's slices may stay in flight
             ring_barrier();
+            // advance the ring: stage st+1's slices are lo[] then the first
+            // S-PF slots just filled; the rest now hold stage st+2's low
+            // slices; stage st's slots are free
+            if (RREG) {
+                int nc[S], nl[PF > 0 ? PF : 1];
+#pragma unroll
+                for (int k = 0; k < S; ++k) nc[k] = k < PF ? lo[k] : fslot[k - PF];
+#pragma unroll
+                for (int k = 0; k < PF; ++k) nl[k] = fslot[S - PF + k];
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    fslot[k] = cur[k];
+                    cur[k] = nc[k];
+                }
+#pragma unroll
+                for (int k = 0; k < PF; ++k) lo[k] = nl[k];
+            }
         }
         // FP64 flush of the QB-wide block: (tier0 + tier1) + tier2, scaled by
         // 2^(e_row + e_col - 16).

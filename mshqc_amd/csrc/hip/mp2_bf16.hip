@@ -194,7 +194,9 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
             const int aa = tid / NC + PARTS * k;
             double x = 0.0;
             if (a0 + aa < v && c0 + cc < nc) x = Z[((long)ip * v + a0 + aa) * ldz + c0 + cc];
-            T[aa][cc] = x;
+            // columns swizzled by row group: the PARTS lanes reading one
+            // column below (rows PER apart) then hit distinct banks
+            T[aa][cc ^ ((aa / PER) & (NC - 1))] = x;
         }
     }
     __syncthreads();
@@ -203,7 +205,7 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
     {
         // PARTS lanes per column, then a shuffle maximum.
         double mx = 0.0;
-        for (int k = 0; k < PER; ++k) mx = fmax(mx, fabs(T[part * PER + k][cc]));
+        for (int k = 0; k < PER; ++k) mx = fmax(mx, fabs(T[part * PER + k][cc ^ (part & (NC - 1))]));
         for (int m = 1; m < PARTS; m <<= 1) mx = fmax(mx, __shfl_xor(mx, m, 64));
         if (part == 0) exs[cc] = block_exponent(mx);
     }
@@ -217,7 +219,7 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             uint16_t m[S];
-            slice_value<S>(T[PER * part + u8 + u][cc], e, m);
+            slice_value<S>(T[PER * part + u8 + u][cc ^ (part & (NC - 1))], e, m);
 #pragma unroll
             for (int p = 0; p < S; ++p) out[p][u] = m[p];
         }

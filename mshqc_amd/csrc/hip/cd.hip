@@ -2,12 +2,14 @@
 // (cholesky_eri.cc:40-203).
 //
 // The ERI matrix M[(ij)][(kl)] = (ij|kl) (n^2 x n^2, row-major) stays in HBM.
-// Each step is two kernels:
-//   pivot : one workgroup scans the diagonal D for its first maximum;
-//           stops the decomposition when it is below the threshold or the
-//           output is full
-//   vector: L_new(ij) = (M[ij][p] - sum_k L_k(ij) L_k(p)) / sqrt(D_p),
-//           D(ij) -= L_new(ij)^2, clamped at 0
+// After one pivot scan, each step is one kernel:
+//   L_new(ij) = (M[ij][p] - sum_k L_k(ij) L_k(p)) / sqrt(D_p),
+//   D(ij) -= L_new(ij)^2, clamped at 0;
+//   each workgroup writes the first maximum of its updated D entries, and
+//   the last workgroup to finish (an arrival counter) reduces those in
+//   index order to the next pivot, stops the decomposition when the maximum
+//   is below the threshold or the output is full, and gathers the pivot's
+//   entries of the earlier vectors
 // with the same operation order as the reference's scalar loop and no fused
 // multiply-adds (contraction off; IEEE division and square root), so the
 // vectors and the pivot sequence are those of the FP64 host algorithm.
@@ -31,6 +33,7 @@ struct CdState {
     int pivot;
     double dpiv;  // D at the pivot when it was chosen
     double dmax;  // largest remaining diagonal at the stop
+    unsigned arrived;  // workgroups of the current step that have finished
 };
 
 __global__ void cd_init_kernel(const double* __restrict__ M, long np, double* __restrict__ D,
@@ -44,39 +47,41 @@ __global__ void cd_init_kernel(const double* __restrict__ M, long np, double* __
         st->pivot = 0;
         st->dpiv = 0.0;
         st->dmax = 0.0;
+        st->arrived = 0;
     }
 }
 
-// First maximum of D (strictly greater wins; ties keep the lower index), as
-// cholesky_eri.cc:155-167 scans it.
-__global__ __launch_bounds__(PIV_THREADS) void cd_pivot_kernel(const double* __restrict__ D, long np,
-                                                               double thr, int cap, const double* __restrict__ L,
-                                                               double* __restrict__ Lp, CdState* st) {
-    __shared__ double sv[PIV_THREADS];
-    __shared__ long si[PIV_THREADS];
-    __shared__ int skip, gather, nv_sh;
-    if (threadIdx.x == 0) {
-        if (st->pending) {
-            st->nvec += 1;
-            st->pending = 0;
-        }
-        skip = st->done;
+// Commit the pending vector, then pick the next pivot from the first
+// maximum dp at index pi: stop at the output bound or below the threshold
+// (cholesky_eri.cc:155-167 and the loop bounds around them).  One thread.
+__device__ __forceinline__ bool decide_pivot(double dp, long pi, double thr, int cap, CdState* st) {
+    if (st->pending) {
+        st->nvec += 1;
+        st->pending = 0;
     }
-    __syncthreads();
-    if (skip) return;
-    double best = -INFINITY;
-    long bi = np;
-    for (long k = threadIdx.x; k < np; k += PIV_THREADS) {
-        const double d = D[k];
-        if (d > best) {
-            best = d;
-            bi = k;
-        }
+    st->dmax = dp;
+    if (st->nvec >= cap) {
+        st->done = 2;  // the reference's loop bound: stop unconverged
+        return false;
     }
-    sv[threadIdx.x] = best;
-    si[threadIdx.x] = bi;
+    if (dp < thr) {
+        st->done = 1;
+        return false;
+    }
+    st->pivot = (int)pi;
+    st->dpiv = dp;
+    st->pending = 1;
+    return true;
+}
+
+// First maximum over a workgroup of (v, i) pairs, strictly greater wins and
+// ties keep the lower index; the result is in sv[0], si[0].
+template <int NT>
+__device__ __forceinline__ void block_first_max(double v, long i, double* sv, long* si) {
+    sv[threadIdx.x] = v;
+    si[threadIdx.x] = i;
     __syncthreads();
-    for (int w = PIV_THREADS / 2; w > 0; w >>= 1) {
+    for (int w = NT / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) {
             const double o = sv[threadIdx.x + w];
             const long oi = si[threadIdx.x + w];
@@ -87,64 +92,106 @@ __global__ __launch_bounds__(PIV_THREADS) void cd_pivot_kernel(const double* __r
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const double dp = sv[0];
-        st->dmax = dp;
-        gather = 0;
-        nv_sh = st->nvec;
-        if (st->nvec >= cap) {
-            st->done = 2;  // the reference's loop bound: stop unconverged
-        } else if (dp < thr) {
-            st->done = 1;
-        } else {
-            st->pivot = (int)si[0];
-            st->dpiv = dp;
-            st->pending = 1;
-            gather = 1;
+}
+
+// The first pivot: the first maximum of the initial diagonal (strictly
+// greater wins; ties keep the lower index), as cholesky_eri.cc:155-167 scans
+// it.  Later pivots come from cd_step_kernel.
+__global__ __launch_bounds__(PIV_THREADS) void cd_pivot_kernel(const double* __restrict__ D, long np,
+                                                               double thr, int cap, CdState* st) {
+    __shared__ double sv[PIV_THREADS];
+    __shared__ long si[PIV_THREADS];
+    double best = -INFINITY;
+    long bi = np;
+    for (long k = threadIdx.x; k < np; k += PIV_THREADS) {
+        const double d = D[k];
+        if (d > best) {
+            best = d;
+            bi = k;
         }
     }
-    __syncthreads();
-    // gather the pivot's entries of the earlier vectors: Lp[k] = L_k(p)
-    if (gather) {
-        const long p = si[0];
-        for (int k = threadIdx.x; k < nv_sh; k += PIV_THREADS) Lp[k] = L[(long)k * np + p];
-    }
+    block_first_max<PIV_THREADS>(best, bi, sv, si);
+    if (threadIdx.x == 0) decide_pivot(sv[0], si[0], thr, cap, st);
 }
 
 // New vector nvec and the diagonal update (cholesky_eri.cc:107-113, 169-203).
 // Loads run UNR vectors ahead of the (sequential, k-ordered) differences.
 constexpr int UNR = 16;
 
-__global__ __launch_bounds__(VEC_THREADS) void cd_vector_kernel(const double* __restrict__ M, long np,
-                                                                double* __restrict__ L, double* __restrict__ D,
-                                                                const double* __restrict__ Lp,
-                                                                const CdState* st) {
+// One step: new vector nvec and the diagonal update (cholesky_eri.cc:107-113,
+// 169-203; loads run UNR vectors ahead of the sequential, k-ordered
+// differences), then the next pivot from the updated diagonal by the last
+// workgroup.
+// bmax/bidx hold one first maximum per workgroup.
+__global__ __launch_bounds__(VEC_THREADS) void cd_step_kernel(const double* __restrict__ M, long np,
+                                                              double* __restrict__ L, double* __restrict__ D,
+                                                              double* __restrict__ Lp, CdState* st, double thr,
+                                                              int cap, double* __restrict__ bmax,
+                                                              long* __restrict__ bidx) {
 #pragma clang fp contract(off)
+    __shared__ double sv[VEC_THREADS];
+    __shared__ long si[VEC_THREADS];
+    __shared__ int last, gather, nv_sh;
     if (st->done || !st->pending) return;
     const int nv = st->nvec;
     const long p = st->pivot;
     const double nf = sqrt(st->dpiv);
     const long ij = (long)blockIdx.x * VEC_THREADS + threadIdx.x;
-    if (ij >= np) return;
-    double val = M[ij * np + p];
-    const double* Lc = L + ij;
-    int k = 0;
-    for (; k + UNR <= nv; k += UNR) {
-        double a[UNR], b[UNR];
+    double d = -INFINITY;
+    if (ij < np) {
+        double val = M[ij * np + p];
+        const double* Lc = L + ij;
+        int k = 0;
+        for (; k + UNR <= nv; k += UNR) {
+            double a[UNR], b[UNR];
 #pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            a[u] = Lc[(long)(k + u) * np];
-            b[u] = Lp[k + u];
+            for (int u = 0; u < UNR; ++u) {
+                a[u] = Lc[(long)(k + u) * np];
+                b[u] = Lp[k + u];
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) val = val - a[u] * b[u];
         }
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) val = val - a[u] * b[u];
+        for (; k < nv; ++k) val = val - Lc[(long)k * np] * Lp[k];
+        if (nf > 1e-12) val = val / nf;
+        L[(long)nv * np + ij] = val;
+        d = D[ij] - val * val;
+        if (d < 0.0) d = 0.0;
+        D[ij] = d;
     }
-    for (; k < nv; ++k) val = val - Lc[(long)k * np] * Lp[k];
-    if (nf > 1e-12) val = val / nf;
-    L[(long)nv * np + ij] = val;
-    double d = D[ij] - val * val;
-    if (d < 0.0) d = 0.0;
-    D[ij] = d;
+    block_first_max<VEC_THREADS>(d, ij < np ? ij : np, sv, si);
+    if (threadIdx.x == 0) {
+        bmax[blockIdx.x] = sv[0];
+        bidx[blockIdx.x] = si[0];
+        __threadfence();
+        last = atomicAdd(&st->arrived, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    // The last workgroup: every other one has written its vector entries,
+    // diagonal and maximum (each fenced before its arrival).
+    __threadfence();
+    double best = -INFINITY;
+    long bi = np;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += VEC_THREADS) {
+        const double v = bmax[b];
+        if (v > best) {
+            best = v;
+            bi = bidx[b];
+        }
+    }
+    __syncthreads();
+    block_first_max<VEC_THREADS>(best, bi, sv, si);
+    if (threadIdx.x == 0) {
+        st->arrived = 0;
+        gather = decide_pivot(sv[0], si[0], thr, cap, st) ? 1 : 0;
+        nv_sh = st->nvec;
+    }
+    __syncthreads();
+    if (gather) {
+        const long pn = si[0];
+        for (int k = threadIdx.x; k < nv_sh; k += VEC_THREADS) Lp[k] = L[(long)k * np + pn];
+    }
 }
 
 }  // namespace
@@ -161,16 +208,20 @@ extern "C" int mshqc_hip_cholesky(mshqc_hip_ctx* ctx, const double* eri, int n, 
     double* D = static_cast<double*>(workspace(ctx, WS_TMP, sizeof(double) * (size_t)np));
     CdState* st = static_cast<CdState*>(workspace(ctx, WS_MISC, sizeof(CdState)));
     double* Lp = static_cast<double*>(workspace(ctx, WS_TMP2, sizeof(double) * (size_t)max_vectors));
-    if (!D || !st || !Lp) return MSHQC_HIP_ERR_ALLOC;
+    const unsigned vgrid = (unsigned)((np + VEC_THREADS - 1) / VEC_THREADS);
+    char* blk = static_cast<char*>(workspace(ctx, WS_PART, (sizeof(double) + sizeof(long)) * (size_t)vgrid + 256));
+    if (!D || !st || !Lp || !blk) return MSHQC_HIP_ERR_ALLOC;
+    double* bmax = reinterpret_cast<double*>(blk);
+    long* bidx = reinterpret_cast<long*>(blk + sizeof(double) * (size_t)vgrid);
     cd_init_kernel<<<(unsigned)((np + 255) / 256), 256, 0, ctx->stream>>>(eri, np, D, st);
     LAUNCH_CHECK(ctx);
-    const unsigned vgrid = (unsigned)((np + VEC_THREADS - 1) / VEC_THREADS);
+    cd_pivot_kernel<<<1, PIV_THREADS, 0, ctx->stream>>>(D, np, threshold, max_vectors, st);
+    LAUNCH_CHECK(ctx);
     CdState h{};
     for (;;) {
         for (int s = 0; s < STEPS_PER_BATCH; ++s) {
-            cd_pivot_kernel<<<1, PIV_THREADS, 0, ctx->stream>>>(D, np, threshold, max_vectors, L, Lp, st);
-            LAUNCH_CHECK(ctx);
-            cd_vector_kernel<<<vgrid, VEC_THREADS, 0, ctx->stream>>>(eri, np, L, D, Lp, st);
+            cd_step_kernel<<<vgrid, VEC_THREADS, 0, ctx->stream>>>(eri, np, L, D, Lp, st, threshold, max_vectors,
+                                                                   bmax, bidx);
             LAUNCH_CHECK(ctx);
         }
         HIP_TRY(ctx, hipMemcpyAsync(&h, st, sizeof(CdState), hipMemcpyDeviceToHost, ctx->stream));

@@ -242,6 +242,22 @@ int mshqc_hip_omp2_opdm(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
                         const double* eps_ob, const double* eps_vb, double* E,
                         double* Goo_a, double* Gvv_a, double* Goo_b, double* Gvv_b);
 
+/*
+ * Two-particle term of the CD-OMP2 orbital gradient for one spin, the term
+ * the reference's w_ai = 2 F_vo leaves out (build_gfock_from_density and its
+ * use at src/mp2/cholesky_omp2.cc:331-369,462-467 form F from the OPDM
+ * only).  With G = dE/dB (DEVICE [Q][o][v], e.g. from mshqc_hip_ump2_grad
+ * in FP64) and the same orbitals' Loo (DEVICE [Q][o][o]) and Lvv (DEVICE
+ * [Q][v][v]):
+ *   w2(c,k) = sum_Q [ sum_a Lvv^Q(c,a) G^Q(k,a) - sum_i G^Q(i,c) Loo^Q(i,k) ]
+ * = dE_MP2/dkappa_ck at fixed denominators, for the rotation that adds
+ * kappa_ck C_c to occupied orbital k and subtracts kappa_ck C_k from
+ * virtual c (the convention in which the reference's 2 F_vo is the SCF
+ * energy's gradient).  w2 is a DEVICE array [v][o], overwritten.
+ */
+int mshqc_hip_omp2_grad2p(mshqc_hip_ctx* ctx, const double* G, const double* Loo,
+                          const double* Lvv, int Q, int o, int v, double* w2);
+
 /* ---------------------------------------------------------------- RMP3 */
 
 /*

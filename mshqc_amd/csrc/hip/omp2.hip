@@ -286,3 +286,67 @@ extern "C" int mshqc_hip_omp2_opdm(mshqc_hip_ctx* ctx, const double* Ba, const d
     hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
     return MSHQC_HIP_OK;
 }
+
+// Two-particle term of the orbital gradient for one spin: with G = dE/dB
+// [Q][o][v] and the same orbitals' Loo [Q][o][o], Lvv [Q][v][v],
+//   w2(c, k) = sum_Q [ sum_a Lvv^Q(c, a) G^Q(k, a) - sum_i G^Q(i, c) Loo^Q(i, k) ]
+// (the derivative of E through B^Q_ia = C_i^T L^Q C_a when C_k gains
+// kappa_ck C_c and C_c loses kappa_ck C_k).  Two batch-reduced FP64 GEMMs
+// over Q; the split partials are summed in a fixed order.
+extern "C" int mshqc_hip_omp2_grad2p(mshqc_hip_ctx* ctx, const double* G, const double* Loo,
+                                     const double* Lvv, int Q, int o, int v, double* w2) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, Q > 0 && o >= 0 && v >= 0, "omp2_grad2p: bad dimensions");
+    CHECK_ARG(ctx, (long)o * v == 0 || (G && Loo && Lvv && w2), "omp2_grad2p: null pointer");
+    if ((long)o * v == 0) return MSHQC_HIP_OK;
+    hipEventRecord(ctx->ev0, ctx->stream);
+    GemmF64 g;  // w2 = sum_Q Lvv_Q G_Q^T
+    g.A = Lvv;
+    g.lda = v;
+    g.sA = (long)v * v;
+    g.X = G;
+    g.ldx = v;
+    g.sX = (long)o * v;
+    g.transX = true;
+    g.C = w2;
+    g.ldc = o;
+    g.M = v;
+    g.N = o;
+    g.K = v;
+    g.batch = Q;
+    g.reduce_batch = true;
+    g.splits = red_splits(Q, v, o);
+    {
+        ProfScope ps(ctx, PROF_GEMM);
+        rc = gemm_f64(ctx, g);
+    }
+    if (rc) return rc;
+    GemmF64 h;  // w2 -= sum_Q G_Q^T Loo_Q
+    h.A = G;
+    h.lda = v;
+    h.sA = (long)o * v;
+    h.transA = true;
+    h.X = Loo;
+    h.ldx = o;
+    h.sX = (long)o * o;
+    h.C = w2;
+    h.ldc = o;
+    h.M = v;
+    h.N = o;
+    h.K = o;
+    h.batch = Q;
+    h.reduce_batch = true;
+    h.splits = red_splits(Q, v, o);
+    h.alpha = -1.0;
+    h.accumulate = true;
+    {
+        ProfScope ps(ctx, PROF_GEMM);
+        rc = gemm_f64(ctx, h);
+    }
+    if (rc) return rc;
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if ((rc = ctx_finish(ctx))) return rc;
+    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+    return MSHQC_HIP_OK;
+}

@@ -347,7 +347,8 @@ PYBIND11_MODULE(_mshqc, m) {
         .def_readwrite("energy_threshold", &CholeskyOMP2Config::energy_threshold)
         .def_readwrite("gradient_threshold", &CholeskyOMP2Config::gradient_threshold)
         .def_readwrite("cholesky_threshold", &CholeskyOMP2Config::cholesky_threshold)
-        .def_readwrite("print_level", &CholeskyOMP2Config::print_level);
+        .def_readwrite("print_level", &CholeskyOMP2Config::print_level)
+        .def_readwrite("two_particle_gradient", &CholeskyOMP2Config::two_particle_gradient);
 
     py::class_<CholeskyOMP2>(m, "CholeskyOMP2")
         .def(py::init<const Molecule&, const BasisSet&, std::shared_ptr<IntegralEngine>, const SCFResult&,

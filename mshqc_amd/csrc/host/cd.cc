@@ -982,6 +982,47 @@ void CholeskyOMP2::rotate(const Mat& w, const Mat& F_mo, int no, int nv, Mat& C,
     P = matmul_nt(Co, Co);
 }
 
+void CholeskyOMP2::add_two_particle_gradient(const double* Ba, const double* Bb, const double* eoa,
+                                             const double* eva, const double* eob, const double* evb,
+                                             bool same_orbitals, Mat (&w)[2]) const {
+    // dE/dB per spin in FP64 (the energy is the UMP2 expression with the
+    // SCF denominators), then w2 = sum_Q Lvv G^T - G^T Loo per spin
+    const int n = nbf_, Q = chol_->n_vectors();
+    std::unique_ptr<device::Buffer> dBa, dBb;
+    if (na_ * va_ > 0) dBa = std::make_unique<device::Buffer>((size_t)Q * na_ * va_);
+    if (nb_ * vb_ > 0) dBb = std::make_unique<device::Buffer>((size_t)Q * nb_ * vb_);
+    double E[3];
+    device::check(mshqc_hip_ump2_grad(device::ctx(), Ba, Bb, Q, na_, va_, nb_, vb_, eoa, eva,
+                                      eob, evb, MSHQC_PREC_FP64, E, dBa ? dBa->get() : nullptr,
+                                      dBb ? dBb->get() : nullptr, nullptr, nullptr, nullptr, nullptr),
+                  "mshqc_hip_ump2_grad");
+    std::unique_ptr<device::Buffer> Loo, Lvv;
+    for (int s = 0; s < 2; ++s) {
+        const int no = s == 0 ? na_ : nb_, nv = s == 0 ? va_ : vb_;
+        if (no * nv == 0) continue;
+        if (s == 0 || !same_orbitals) {
+            const Mat& C = s == 0 ? scf_.C_alpha : scf_.C_beta;
+            auto dCo = to_device(left_cols(C, no));
+            auto dCv = to_device(right_cols(C, nv));
+            Loo = std::make_unique<device::Buffer>((size_t)Q * no * no);
+            Lvv = std::make_unique<device::Buffer>((size_t)Q * nv * nv);
+            device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, dCo->get(), no,
+                                              dCo->get(), no, Loo->get()),
+                          "mshqc_hip_transform");
+            device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, dCv->get(), nv,
+                                              dCv->get(), nv, Lvv->get()),
+                          "mshqc_hip_transform");
+        }
+        device::Buffer dw((size_t)nv * no);
+        device::check(mshqc_hip_omp2_grad2p(device::ctx(), (s == 0 ? dBa : dBb)->get(), Loo->get(), Lvv->get(), Q,
+                                            no, nv, dw.get()),
+                      "mshqc_hip_omp2_grad2p");
+        Mat w2(nv, no);
+        dw.download(w2.data(), w2.size());
+        w[s] = add(w[s], w2);
+    }
+}
+
 OMP2Result CholeskyOMP2::compute() {
     // cholesky_omp2.cc:427-513
     const int n = nbf_, Q = chol_->n_vectors();
@@ -1059,6 +1100,11 @@ OMP2Result CholeskyOMP2::compute() {
             for (int a = 0; a < nv; ++a)
                 for (int i = 0; i < no; ++i) w[s](a, i) = 2.0 * F_mo[s](no + a, i);
             g2 += dot(w[s], w[s]);
+        }
+        if (config_.two_particle_gradient) {
+            add_two_particle_gradient(Ba ? Ba->get() : nullptr, pBb, deoa->get(), deva->get(), deob->get(), devb->get(),
+                                      same_orbitals, w);
+            g2 = dot(w[0], w[0]) + dot(w[1], w[1]);
         }
         const double g_norm = std::sqrt(g2);
         if (config_.print_level > 0)

@@ -373,6 +373,10 @@ struct CholeskyOMP2Config {
     double gradient_threshold = 1e-5;
     double cholesky_threshold = 1e-6;
     int print_level = 1;
+    // Not in the reference (default off, which is the reference's loop):
+    // add the two-particle term dE_MP2/dkappa (mshqc_hip_omp2_grad2p) to
+    // each spin's w_ai = 2 F_vo, in the gradient norm and in the rotation.
+    bool two_particle_gradient = false;
 };
 
 // cholesky_omp2.h / cholesky_omp2.cc: MP2 amplitudes and one-particle density
@@ -399,6 +403,9 @@ private:
     int nbf_, na_, nb_, va_, vb_;
 
     void rotate(const Mat& w, const Mat& F_mo, int no, int nv, Mat& C, Mat& P) const;
+    // w[s] += the two-particle term of spin s (device B and orbital energies)
+    void add_two_particle_gradient(const double* Ba, const double* Bb, const double* eoa, const double* eva,
+                                   const double* eob, const double* evb, bool same_orbitals, Mat (&w)[2]) const;
 };
 
 }  // namespace mshqc

@@ -45,6 +45,7 @@ _SIGS = {
                            ctypes.POINTER(_c_int), _D],
     "mshqc_hip_omp2_opdm": [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                             _D, _vp, _vp, _vp, _vp],
+    "mshqc_hip_omp2_grad2p": [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp],
     "mshqc_hip_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp],
     "mshqc_hip_rmp2_energy": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D],
     "mshqc_hip_rmp2_grad": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D,
@@ -320,6 +321,18 @@ class Context:
             E, *[self._ptr(g) for g in G])
         self._check(rc, "mshqc_hip_omp2_opdm")
         return tuple(E), tuple(G)
+
+    def omp2_grad2p(self, G, Loo, Lvv):
+        """Two-particle term w2 [v, o] of the OMP2 orbital gradient of one spin,
+        from G = dE/dB [Q, o, v] and the same orbitals' Loo [Q, o, o], Lvv [Q, v, v]."""
+        torch = self._torch
+        Q, o, v = G.shape
+        w2 = torch.zeros((v, o), dtype=torch.float64, device=G.device)
+        self._follow_stream()
+        rc = lib().mshqc_hip_omp2_grad2p(self.h, self._ptr(G, name="G"), self._ptr(Loo, (Q, o, o), "Loo"),
+                                         self._ptr(Lvv, (Q, v, v), "Lvv"), Q, o, v, self._ptr(w2))
+        self._check(rc, "mshqc_hip_omp2_grad2p")
+        return w2
 
     def rmp3(self, B, Loo, Lvv, t1, eps_o, eps_v, want_t2=False):
         """(E3, t2_2 or None) of CD-RMP3 from B [Q,o,v], Loo [Q,o,o], Lvv [Q,v,v], t1 [o,o,v,v]."""

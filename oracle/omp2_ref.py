@@ -17,6 +17,13 @@ INFRASTRUCTURE ONLY (tiny sizes).
   build's own code (via the pinned pair oracle), a one-electron system
   stopping at iteration 1 with the SCF energy (w = 2 F_vo is the SCF
   orbital gradient), and orthonormal rotated orbitals (tests/test_omp2.py).
+  With ``two_particle=True`` (not in the reference) each spin's w also gets
+  ``two_particle_grad``.
+* ``two_particle_grad``  the two-particle term the reference's w = 2 F_vo
+  leaves out (cholesky_omp2.cc:331-369 build F from the OPDM only):
+  w2(c, k) = dE_MP2/dkappa_ck at fixed denominators, from dE/dB and the
+  oo / vv blocks.  Pinned by central differences of the MP2 energy under
+  the orbital rotation C -> C expm(kappa) (tests/test_omp2.py).
 """
 from __future__ import annotations
 
@@ -75,15 +82,23 @@ def opdm(taa, tbb, tab):
     return Goo_a, Gvv_a, Goo_b, Gvv_b
 
 
+def two_particle_grad(dB, Loo, Lvv):
+    """w2[c, k] = sum_Q [sum_a Lvv[Q, c, a] dB[Q, k, a] - sum_i dB[Q, i, c] Loo[Q, i, k]]:
+    B^Q_ia = C_i^T L^Q C_a changes by sum_c kappa_ci Lvv^Q(c, a) - sum_k kappa_ak
+    Loo^Q(i, k) when occupied k gains kappa_ck C_c and virtual c loses
+    kappa_ck C_k, and the chain rule through dE/dB gives w2."""
+    return np.einsum("qca,qka->ck", Lvv, dB) - np.einsum("qic,qik->ck", dB, Loo)
+
+
 def cd_omp2(H, L, C_a, C_b, P_a, P_b, eps_a, eps_b, na, nb, e_scf, max_iterations=50,
-            energy_threshold=1e-6, gradient_threshold=1e-5):
+            energy_threshold=1e-6, gradient_threshold=1e-5, two_particle=False):
     """The CD-OMP2 loop of cholesky_omp2.cc:427-513 with numpy and the C
     oracle's transform and J/K.  Returns dict(energy_total, e_mp2, e_ss, e_os,
     C_a, C_b, iterations, converged, trace) where trace lists (E_total,
     E_mp2, g_norm) per iteration."""
     import scipy.linalg as sl
 
-    from . import jk as _jk, transform as _transform
+    from . import jk as _jk, transform as _transform, ump2 as _ump2
 
     n = H.shape[0]
     va, vb = n - na, n - nb
@@ -135,6 +150,12 @@ def cd_omp2(H, L, C_a, C_b, P_a, P_b, eps_a, eps_b, na, nb, e_scf, max_iteration
         F_b = C_b.T @ (H + J - K[1]) @ C_b
         w_a = 2.0 * F_a[na:, :na]
         w_b = 2.0 * F_b[nb:, :nb]
+        if two_particle:
+            _, (dBa, dBb, *_) = _ump2(Ba, Bb, eoa, eva, eob, evb, grad=True)
+            w_a = w_a + two_particle_grad(dBa, _transform(L, C_a[:, :na], C_a[:, :na]),
+                                          _transform(L, C_a[:, na:], C_a[:, na:]))
+            w_b = w_b + two_particle_grad(dBb, _transform(L, C_b[:, :nb], C_b[:, :nb]),
+                                          _transform(L, C_b[:, nb:], C_b[:, nb:]))
         g_norm = float(np.sqrt(np.sum(w_a * w_a) + np.sum(w_b * w_b)))
         trace.append((e_curr, e_mp2, g_norm))
         if g_norm < gradient_threshold and abs(e_curr - e_prev) < energy_threshold:

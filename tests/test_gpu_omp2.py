@@ -66,6 +66,22 @@ def test_omp2_opdm_identical_spins(ctx):
     check_close(to_np(G[3]), to_np(G[1]), 1e-13, "Gvv_b vs Gvv_a")
 
 
+@pytest.mark.parametrize("Q,o,v,seed", [(23, 5, 11, 1), (31, 1, 20, 2), (17, 6, 1, 3), (40, 12, 130, 4),
+                                          (1, 4, 6, 5)])
+def test_omp2_grad2p_matches_oracle(ctx, Q, o, v, seed):
+    """Two-particle orbital-gradient term (batch-reduced GEMMs over Q; Q = 1
+    takes the unsplit path) against the oracle's einsum form, which the CPU
+    tests pin by finite differences of the MP2 energy."""
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((Q, o, v))
+    Loo = rng.standard_normal((Q, o, o))
+    Lvv = rng.standard_normal((Q, v, v))
+    Loo, Lvv = Loo + Loo.transpose(0, 2, 1), Lvv + Lvv.transpose(0, 2, 1)
+    print(f"omp2_grad2p Q={Q} o={o} v={v}", flush=True)
+    w2 = ctx.omp2_grad2p(*to_dev(G, Loo, Lvv))
+    check_close(to_np(w2), omp2_ref.two_particle_grad(G, Loo, Lvv), 1e-12, "w2")
+
+
 @pytest.fixture(scope="module")
 def m(ctx):
     from mshqc_amd import native_module
@@ -88,12 +104,13 @@ def _setup(m, atoms, basis, mult):
 
 
 def _compare(m, mol, b, eng, chol, L, H, scf, cfg):
-    print(f"CD-OMP2 n={b.n_basis_functions()} na={scf.n_occ_alpha} nb={scf.n_occ_beta}", flush=True)
+    print(f"CD-OMP2 n={b.n_basis_functions()} na={scf.n_occ_alpha} nb={scf.n_occ_beta} "
+          f"two_particle={cfg.two_particle_gradient}", flush=True)
     res = m.CholeskyOMP2(mol, b, eng, scf, cfg, chol).compute()
     ref = omp2_ref.cd_omp2(H, L, scf.C_alpha, scf.C_beta, scf.P_alpha, scf.P_beta, scf.orbital_energies_alpha,
                            scf.orbital_energies_beta, scf.n_occ_alpha, scf.n_occ_beta, scf.energy_total,
                            max_iterations=cfg.max_iterations, energy_threshold=cfg.energy_threshold,
-                           gradient_threshold=cfg.gradient_threshold)
+                           gradient_threshold=cfg.gradient_threshold, two_particle=cfg.two_particle_gradient)
     print(f"  E {res.energy_total!r} oracle {ref['energy_total']!r} iters {res.iterations}/{ref['iterations']}",
           flush=True)
     assert res.iterations == ref["iterations"] and res.converged == ref["converged"]
@@ -136,5 +153,34 @@ def test_cd_omp2_from_uhf(m):
     cfg.print_level = 0
     cfg.energy_threshold = 1e-9
     cfg.gradient_threshold = 1e-7
+    res = _compare(m, mol, b, eng, chol, L, H, scf, cfg)
+    assert res.converged
+
+
+@pytest.mark.parametrize("mult", [1, 2])
+def test_cd_omp2_two_particle(m, mult):
+    """The loop with the two-particle gradient term (not in the reference):
+    H4 from CD-RHF and H3 doublet from CD-UHF, cc-pVDZ, against the oracle's
+    loop with the same term."""
+    nat = 4 if mult == 1 else 3
+    mol, b, eng, chol, L, H = _setup(m, [(1, 0.0, 0.0, 1.4 * k) for k in range(nat)], "cc-pvdz", mult)
+    if mult == 1:
+        rcfg = m.CholeskyRHFConfig()
+        rcfg.print_level = 0
+        scf = m.CholeskyRHF(mol, b, eng, rcfg, chol).compute()
+    else:
+        ucfg = m.CholeskyUHFConfig()
+        ucfg.print_level = 0
+        ucfg.energy_threshold = 1e-11
+        ucfg.max_iterations = 300
+        uhf = m.CholeskyUHF(mol, b, eng, 2, 1, ucfg)
+        uhf.set_cholesky_vectors(chol.get_L_vectors())
+        scf = uhf.compute()
+    cfg = m.CholeskyOMP2Config()
+    cfg.print_level = 0
+    cfg.two_particle_gradient = True
+    if mult == 2:
+        cfg.energy_threshold = 1e-9
+        cfg.gradient_threshold = 1e-7
     res = _compare(m, mol, b, eng, chol, L, H, scf, cfg)
     assert res.converged

@@ -98,6 +98,69 @@ def test_one_electron_stops_at_scf(m):
     assert r["trace"][0][2] < 1e-9
 
 
+def _rotated(C, o, c, k, h):
+    # C expm(K): occupied k gains h C_c, virtual c loses h C_k (to first order)
+    n = C.shape[0]
+    K = np.zeros((n, n))
+    K[o + c, k] = h
+    K[k, o + c] = -h
+    return C @ sl.expm(K)
+
+
+@pytest.mark.parametrize("Q,n,na,nb,seed", [(30, 10, 3, 2, 3), (12, 7, 1, 1, 4), (25, 9, 4, 0, 5)])
+def test_two_particle_grad_finite_differences(Q, n, na, nb, seed):
+    """The two-particle term is the MP2 energy's derivative under the orbital
+    rotation at fixed denominators: central differences of the energy of
+    B(C expm(kappa)) (step 1e-4, error ~h^2) against the analytic term, for
+    entries of both spins; a dropped term or a transposed block fails."""
+    from mshqc_amd import synthetic as syn
+
+    rng = np.random.default_rng(seed)
+    L = syn.gen_L(Q, n, rng)
+    Ca = sl.qr(rng.standard_normal((n, n)))[0]
+    Cb = sl.qr(rng.standard_normal((n, n)))[0]
+    eoa, eva = syn.gen_eps(na, n - na, rng)
+    eob, evb = syn.gen_eps(nb, n - nb, rng)
+
+    def B(C, o):
+        return oracle.transform(L, C[:, :o], C[:, o:]) if o else np.zeros((Q, 0, n))
+
+    def E(Ca, Cb):
+        return sum(oracle.ump2(B(Ca, na), B(Cb, nb), eoa, eva, eob, evb))
+
+    _, (dBa, dBb, *_) = oracle.ump2(B(Ca, na), B(Cb, nb), eoa, eva, eob, evb, grad=True)
+    h = 1e-4
+    for spin, C, o, dB in ((0, Ca, na, dBa), (1, Cb, nb, dBb)):
+        if o == 0:
+            continue
+        w2 = omp2_ref.two_particle_grad(dB, oracle.transform(L, C[:, :o], C[:, :o]),
+                                        oracle.transform(L, C[:, o:], C[:, o:]))
+        assert w2.shape == (n - o, o)
+        for c, k in {(0, 0), (n - o - 1, o - 1), (min(2, n - o - 1), min(1, o - 1))}:
+            if spin == 0:
+                fd = (E(_rotated(Ca, o, c, k, h), Cb) - E(_rotated(Ca, o, c, k, -h), Cb)) / (2 * h)
+            else:
+                fd = (E(Ca, _rotated(Cb, o, c, k, h)) - E(Ca, _rotated(Cb, o, c, k, -h))) / (2 * h)
+            assert abs(fd - w2[c, k]) <= 1e-6 * abs(w2[c, k]) + 1e-11, (spin, c, k, fd, w2[c, k])
+
+
+def test_two_particle_loop_converges(m):
+    """H4 chain, cc-pVDZ: with the two-particle term the loop still converges
+    (to a lower MP2 energy: the rotation now also follows the pair term), and
+    its first iteration is the reference loop's first iteration plus the term."""
+    mol, S, H, L = system(m, [(1, 0.0, 0.0, 1.4 * k) for k in range(4)], "cc-pvdz", thr=1e-6)
+    g = rhf_guess(mol, S, H, L, 2)
+    args = (H, L, g["C_a"], g["C_b"], g["P_a"], g["P_b"], g["eps_a"], g["eps_b"], 2, 2, g["e_scf"])
+    ref = omp2_ref.cd_omp2(*args)
+    r = omp2_ref.cd_omp2(*args, two_particle=True)
+    assert ref["converged"] and r["converged"]
+    assert r["trace"][0][:2] == ref["trace"][0][:2]
+    assert r["trace"][0][2] > ref["trace"][0][2]
+    assert r["trace"][-1][2] < 1e-5
+    assert r["e_mp2"] < ref["e_mp2"]
+    assert m.CholeskyOMP2Config().two_particle_gradient is False
+
+
 @pytest.mark.parametrize("scale", [1e-3, 0.3, 4.0])
 def test_host_expm(m, scale):
     rng = np.random.default_rng(int(scale * 1000))

@@ -392,12 +392,16 @@ const Shell& BasisSet::shell(int i) const {
 
 namespace {
 
-constexpr int LMAX = 1;                 // per shell (s, p)
+constexpr int LMAX = 4;                 // per shell (s .. g)
 constexpr int TMAX = 4 * LMAX;          // Hermite index bound in an ERI
 
-// Cartesian exponents of component c of a shell with angular momentum L
-// (order x, y, z for L = 1).
+int n_cart(int L) { return (L + 1) * (L + 2) / 2; }
+
+// Cartesian exponents of component c of a shell with angular momentum L, in
+// the order x^L, x^(L-1) y, x^(L-1) z, ..., z^L (x, y, z for L = 1; xx, xy,
+// xz, yy, yz, zz for L = 2).
 void cart(int L, int c, int& lx, int& ly, int& lz) {
+    lx = ly = lz = 0;
     int idx = 0;
     for (int a = L; a >= 0; --a)
         for (int b = L - a; b >= 0; --b) {
@@ -411,14 +415,28 @@ void cart(int L, int c, int& lx, int& ly, int& lz) {
         }
 }
 
+int cart_index(int L, int lx, int ly) {
+    int idx = 0;
+    for (int a = L; a > lx; --a) idx += L - a + 1;
+    return idx + (L - lx - ly);
+}
+
+double binom(int n, int k) {
+    if (k < 0 || k > n) return 0.0;
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
 // Boys function F_n(T), n = 0..nmax: series for the top order, downward
-// recursion below (F_n = (2T F_{n+1} + e^-T) / (2n+1)); asymptotic form for
-// large T.
+// recursion below (F_n = (2T F_{n+1} + e^-T) / (2n+1)); the asymptotic form
+// (2n-1)!! / 2^(n+1) sqrt(pi / T^(2n+1)) once the neglected e^-T terms are
+// below ~1e-14 of F_n, which needs larger T for larger n.
 void boys(int nmax, double T, double* F) {
-    if (T < 35.0) {
+    if (T < 40.0 + 2.0 * nmax) {
         const double e = std::exp(-T);
         double term = 1.0 / (2 * nmax + 1), sum = term;
-        for (int k = 1; k < 400; ++k) {
+        for (int k = 1; k < 600; ++k) {
             term *= 2.0 * T / (2 * nmax + 2 * k + 1);
             sum += term;
             if (term < 1e-17 * sum) break;
@@ -503,12 +521,79 @@ void hermite_coulomb(int nmax, double p, double X, double Y, double Z, RTab& out
             for (int v = 0; v <= nmax - t - u; ++v) out.R[t][u][v] = Rn[0][t][u][v];
 }
 
+// Real solid harmonics S_lm (l >= 2; m = -l..l, the sine type for m < 0) over
+// a shell's Cartesian components, from the closed form in Helgaker,
+// Jorgensen and Olsen, "Molecular Electronic-Structure Theory", eq. 6.4.47-50,
+// each row scaled so that S_lm is normalised when every Cartesian component
+// carries the normalisation of x^l (the shells' coefficient convention): for
+// one exponent, <x^a y^b z^c | x^a' y^b' z^c'> is proportional to
+// (a+a'-1)!! (b+b'-1)!! (c+c'-1)!!.  p shells stay Cartesian (x, y, z).
+struct SphTab {
+    std::vector<double> T[LMAX + 1];  // [(2l+1)][n_cart(l)]
+    SphTab() {
+        for (int l = 2; l <= LMAX; ++l) {
+            const int nc = n_cart(l);
+            T[l].assign((size_t)(2 * l + 1) * nc, 0.0);
+            for (int m = -l; m <= l; ++m) {
+                double* row = &T[l][(size_t)(m + l) * nc];
+                const int am = std::abs(m), km = m < 0 ? 1 : 0;
+                for (int t = 0; t <= (l - am) / 2; ++t)
+                    for (int u = 0; u <= t; ++u)
+                        for (int k = km; k <= am; k += 2) {
+                            const int ex = 2 * t + am - 2 * u - k, ey = 2 * u + k;
+                            const double sgn = ((t + (k - km) / 2) & 1) ? -1.0 : 1.0;
+                            row[cart_index(l, ex, ey)] += sgn * std::pow(0.25, t) * binom(l, t) *
+                                                          binom(l - t, am + t) * binom(t, u) * binom(am, k);
+                        }
+                double nrm = 0.0;
+                for (int a = 0; a < nc; ++a)
+                    for (int b = 0; b < nc; ++b) {
+                        int ax, ay, az, bx, by, bz;
+                        cart(l, a, ax, ay, az);
+                        cart(l, b, bx, by, bz);
+                        if (row[a] == 0.0 || row[b] == 0.0 || ((ax + bx) | (ay + by) | (az + bz)) & 1) continue;
+                        nrm += row[a] * row[b] * double_factorial(ax + bx - 1) * double_factorial(ay + by - 1) *
+                               double_factorial(az + bz - 1);
+                    }
+                const double s = 1.0 / std::sqrt(nrm / double_factorial(2 * l - 1));
+                for (int a = 0; a < nc; ++a) row[a] *= s;
+            }
+        }
+    }
+};
+
+const SphTab& sph_tab() {
+    static const SphTab t;
+    return t;
+}
+
+// Cartesian -> shell-function matrix of a shell ([n_functions][n_cart]), or
+// null when the shell's functions are its Cartesian components.
+const double* sph_rows(const Shell& s) {
+    return (s.is_spherical() && s.l() >= 2) ? sph_tab().T[s.l()].data() : nullptr;
+}
+
+// out[f][rest] = sum_c T[f][c] in[c][rest] along one index of a row-major
+// block [outer][nc][rest] (T null: a copy).
+void to_shell_index(const double* T, int nf, int nc, long outer, long rest, const double* in, double* out) {
+    for (long o = 0; o < outer; ++o)
+        for (int f = 0; f < nf; ++f) {
+            double* dst = out + (o * nf + f) * rest;
+            for (long r = 0; r < rest; ++r) dst[r] = 0.0;
+            for (int c = 0; c < nc; ++c) {
+                const double w = T ? T[(size_t)f * nc + c] : (f == c ? 1.0 : 0.0);
+                if (w == 0.0) continue;
+                const double* src = in + (o * nc + c) * rest;
+                for (long r = 0; r < rest; ++r) dst[r] += w * src[r];
+            }
+        }
+}
+
 void check_shells(const BasisSet& b) {
     for (const auto& s : b.shells())
         if (s.l() > LMAX)
             throw std::runtime_error(
-                "IntegralEngine: shells above L=1 are not supported by this build's integral "
-                "engine (the reference uses libint2, which is not available here)");
+                "IntegralEngine: shells above L=4 (g) are not supported by this build's integral engine");
 }
 
 }  // namespace
@@ -533,8 +618,9 @@ Mat one_body(const BasisSet& basis, const std::vector<int>& offset, int nbf, con
     for (int s1 = 0; s1 < ns; ++s1)
         for (int s2 = 0; s2 <= s1; ++s2) {
             const Shell &A = sh[s1], &B = sh[s2];
-            const int LA = A.l(), LB = B.l();
+            const int LA = A.l(), LB = B.l(), ncA = n_cart(LA), ncB = n_cart(LB);
             const auto &PA = A.position(), &PB = B.position();
+            std::vector<double> blk((size_t)ncA * ncB, 0.0);
             for (const auto& pa : A.primitives())
                 for (const auto& pb : B.primitives()) {
                     const double a = pa.exponent, b = pb.exponent, p = a + b;
@@ -545,8 +631,8 @@ Mat one_body(const BasisSet& basis, const std::vector<int>& offset, int nbf, con
                     const double cc = pa.coefficient * pb.coefficient;
                     const double s3 = std::pow(M_PI / p, 1.5);
                     RTab R;
-                    for (int ca = 0; ca < A.n_functions(); ++ca)
-                        for (int cb = 0; cb < B.n_functions(); ++cb) {
+                    for (int ca = 0; ca < ncA; ++ca)
+                        for (int cb = 0; cb < ncB; ++cb) {
                             int ax, ay, az, bx, by, bz;
                             cart(LA, ca, ax, ay, az);
                             cart(LB, cb, bx, by, bz);
@@ -577,15 +663,77 @@ Mat one_body(const BasisSet& basis, const std::vector<int>& offset, int nbf, con
                                     val -= at.atomic_number * 2.0 * M_PI / p * s;
                                 }
                             }
-                            M(offset[s1] + ca, offset[s2] + cb) += cc * val;
+                            blk[(size_t)ca * ncB + cb] += cc * val;
                         }
                 }
-            if (s1 != s2)
-                for (int ca = 0; ca < A.n_functions(); ++ca)
-                    for (int cb = 0; cb < B.n_functions(); ++cb)
-                        M(offset[s2] + cb, offset[s1] + ca) = M(offset[s1] + ca, offset[s2] + cb);
+            // Cartesian block -> shell functions, index A then index B
+            const int na = A.n_functions(), nb = B.n_functions();
+            std::vector<double> t1((size_t)na * ncB), t2((size_t)nb * na);
+            to_shell_index(sph_rows(A), na, ncA, 1, ncB, blk.data(), t1.data());
+            // transpose to [ncB][na], transform B, giving [nb][na]
+            std::vector<double> tt((size_t)ncB * na);
+            for (int f = 0; f < na; ++f)
+                for (int cb = 0; cb < ncB; ++cb) tt[(size_t)cb * na + f] = t1[(size_t)f * ncB + cb];
+            to_shell_index(sph_rows(B), nb, ncB, 1, na, tt.data(), t2.data());
+            for (int fa = 0; fa < na; ++fa)
+                for (int fb = 0; fb < nb; ++fb) {
+                    const double v = t2[(size_t)fb * na + fa];
+                    M(offset[s1] + fa, offset[s2] + fb) = v;
+                    M(offset[s2] + fb, offset[s1] + fa) = v;
+                }
         }
     return M;
+}
+
+// Hermite coefficients of one primitive pair's Cartesian products: for
+// product (a, b) the nonzero E^a_t E^b_u E^c_v, t+u+v indexed in a table of
+// the pair's Hermite functions.
+struct PairHerm {
+    int nh = 0;                    // Hermite functions t+u+v <= La+Lb
+    std::vector<int> tuv;          // [nh][3]
+    std::vector<int> start;        // [ncart pairs + 1] into idx / coef
+    std::vector<int> idx;
+    std::vector<double> coef;
+};
+
+int hidx(int L, int t, int u, int v) {
+    // flat index of (t, u, v), t+u+v <= L, ordered by t, then u, then v
+    int k = 0;
+    for (int tt = 0; tt < t; ++tt) k += (L - tt + 1) * (L - tt + 2) / 2;
+    for (int uu = 0; uu < u; ++uu) k += L - t - uu + 1;
+    return k + v;
+}
+
+void pair_herm(const Shell& A, const Shell& B, const Herm& ex, const Herm& ey, const Herm& ez, PairHerm& ph) {
+    const int LA = A.l(), LB = B.l(), L = LA + LB, ncA = n_cart(LA), ncB = n_cart(LB);
+    ph.nh = (L + 1) * (L + 2) * (L + 3) / 6;
+    ph.tuv.resize((size_t)3 * ph.nh);
+    for (int t = 0; t <= L; ++t)
+        for (int u = 0; u <= L - t; ++u)
+            for (int v = 0; v <= L - t - u; ++v) {
+                const int k = hidx(L, t, u, v);
+                ph.tuv[3 * k] = t;
+                ph.tuv[3 * k + 1] = u;
+                ph.tuv[3 * k + 2] = v;
+            }
+    ph.start.assign((size_t)ncA * ncB + 1, 0);
+    ph.idx.clear();
+    ph.coef.clear();
+    for (int ca = 0; ca < ncA; ++ca)
+        for (int cb = 0; cb < ncB; ++cb) {
+            int ax, ay, az, bx, by, bz;
+            cart(LA, ca, ax, ay, az);
+            cart(LB, cb, bx, by, bz);
+            for (int t = 0; t <= ax + bx; ++t)
+                for (int u = 0; u <= ay + by; ++u)
+                    for (int v = 0; v <= az + bz; ++v) {
+                        const double e = ex.E[ax][bx][t] * ey.E[ay][by][u] * ez.E[az][bz][v];
+                        if (e == 0.0) continue;
+                        ph.idx.push_back(hidx(L, t, u, v));
+                        ph.coef.push_back(e);
+                    }
+            ph.start[(size_t)ca * ncB + cb + 1] = (int)ph.idx.size();
+        }
 }
 }  // namespace
 
@@ -604,14 +752,17 @@ ERITensor IntegralEngine::compute_eri() const {
     const long np = (long)pairs.size();
 #pragma omp parallel for schedule(dynamic)
     for (long p1 = 0; p1 < np; ++p1) {
+        PairHerm bra, ket;
+        std::vector<double> W, buf, tmp;
         for (long p2 = 0; p2 <= p1; ++p2) {
             const int sa = pairs[p1].first, sb = pairs[p1].second, sc = pairs[p2].first,
                       sd = pairs[p2].second;
             const Shell &A = sh[sa], &B = sh[sb], &C = sh[sc], &D = sh[sd];
-            const int na = A.n_functions(), nb = B.n_functions(), nc = C.n_functions(),
-                      nd = D.n_functions();
-            double buf[3][3][3][3] = {};
+            const int ncA = n_cart(A.l()), ncB = n_cart(B.l()), ncC = n_cart(C.l()), ncD = n_cart(D.l());
+            const long nab = (long)ncA * ncB, ncd = (long)ncC * ncD;
+            buf.assign((size_t)(nab * ncd), 0.0);
             const auto &RA = A.position(), &RB = B.position(), &RC = C.position(), &RD = D.position();
+            const int Lab = A.l() + B.l(), nmax = Lab + C.l() + D.l();
             for (const auto& pa : A.primitives())
                 for (const auto& pb : B.primitives()) {
                     const double a = pa.exponent, b = pb.exponent, p = a + b;
@@ -619,6 +770,7 @@ ERITensor IntegralEngine::compute_eri() const {
                     hermite(a, b, RA[0] - RB[0], A.l(), B.l(), ex);
                     hermite(a, b, RA[1] - RB[1], A.l(), B.l(), ey);
                     hermite(a, b, RA[2] - RB[2], A.l(), B.l(), ez);
+                    pair_herm(A, B, ex, ey, ez, bra);
                     const double P[3] = {(a * RA[0] + b * RB[0]) / p, (a * RA[1] + b * RB[1]) / p,
                                          (a * RA[2] + b * RB[2]) / p};
                     for (const auto& pc : C.primitives())
@@ -628,54 +780,53 @@ ERITensor IntegralEngine::compute_eri() const {
                             hermite(c, d, RC[0] - RD[0], C.l(), D.l(), fx);
                             hermite(c, d, RC[1] - RD[1], C.l(), D.l(), fy);
                             hermite(c, d, RC[2] - RD[2], C.l(), D.l(), fz);
+                            pair_herm(C, D, fx, fy, fz, ket);
                             const double Qc[3] = {(c * RC[0] + d * RD[0]) / q, (c * RC[1] + d * RD[1]) / q,
                                                   (c * RC[2] + d * RD[2]) / q};
                             const double alpha = p * q / (p + q);
                             const double pref = 2.0 * std::pow(M_PI, 2.5) / (p * q * std::sqrt(p + q)) *
                                                 pa.coefficient * pb.coefficient * pc.coefficient *
                                                 pd.coefficient;
-                            const int nmax = A.l() + B.l() + C.l() + D.l();
                             RTab R;
                             hermite_coulomb(nmax, alpha, P[0] - Qc[0], P[1] - Qc[1], P[2] - Qc[2], R);
-                            for (int ia = 0; ia < na; ++ia)
-                                for (int ib = 0; ib < nb; ++ib)
-                                    for (int ic = 0; ic < nc; ++ic)
-                                        for (int id = 0; id < nd; ++id) {
-                                            int ax, ay, az, bx, by, bz, cx, cy, cz, dx, dy, dz;
-                                            cart(A.l(), ia, ax, ay, az);
-                                            cart(B.l(), ib, bx, by, bz);
-                                            cart(C.l(), ic, cx, cy, cz);
-                                            cart(D.l(), id, dx, dy, dz);
-                                            double s = 0.0;
-                                            for (int t = 0; t <= ax + bx; ++t)
-                                                for (int u = 0; u <= ay + by; ++u)
-                                                    for (int v = 0; v <= az + bz; ++v) {
-                                                        const double e1 = ex.E[ax][bx][t] * ey.E[ay][by][u] *
-                                                                          ez.E[az][bz][v];
-                                                        if (e1 == 0.0) continue;
-                                                        for (int tt = 0; tt <= cx + dx; ++tt)
-                                                            for (int uu = 0; uu <= cy + dy; ++uu)
-                                                                for (int vv = 0; vv <= cz + dz; ++vv) {
-                                                                    const double e2 = fx.E[cx][dx][tt] *
-                                                                                      fy.E[cy][dy][uu] *
-                                                                                      fz.E[cz][dz][vv];
-                                                                    const double sg =
-                                                                        ((tt + uu + vv) & 1) ? -1.0 : 1.0;
-                                                                    s += e1 * sg * e2 *
-                                                                         R.R[t + tt][u + uu][v + vv];
-                                                                }
-                                                    }
-                                            buf[ia][ib][ic][id] += pref * s;
-                                        }
+                            // W[cd][h] = sum_{ket Hermite k} (-1)^|k| E^cd_k R[h + k] for every bra
+                            // Hermite function h, then buf[ab][cd] += pref sum_h E^ab_h W[cd][h]
+                            W.assign((size_t)(ncd * bra.nh), 0.0);
+                            for (long cd = 0; cd < ncd; ++cd) {
+                                double* w = &W[(size_t)(cd * bra.nh)];
+                                for (int e = ket.start[cd]; e < ket.start[cd + 1]; ++e) {
+                                    const int k = ket.idx[e];
+                                    const int tt = ket.tuv[3 * k], uu = ket.tuv[3 * k + 1], vv = ket.tuv[3 * k + 2];
+                                    const double f = ((tt + uu + vv) & 1) ? -ket.coef[e] : ket.coef[e];
+                                    for (int h = 0; h < bra.nh; ++h)
+                                        w[h] += f * R.R[bra.tuv[3 * h] + tt][bra.tuv[3 * h + 1] + uu]
+                                                       [bra.tuv[3 * h + 2] + vv];
+                                }
+                            }
+                            for (long ab = 0; ab < nab; ++ab) {
+                                double* out = &buf[(size_t)(ab * ncd)];
+                                for (int e = bra.start[ab]; e < bra.start[ab + 1]; ++e) {
+                                    const double g = pref * bra.coef[e];
+                                    const double* w = &W[bra.idx[e]];
+                                    for (long cd = 0; cd < ncd; ++cd) out[cd] += g * w[cd * bra.nh];
+                                }
+                            }
                         }
                 }
+            // Cartesian quartet -> shell functions, one index at a time
+            const int na = A.n_functions(), nb = B.n_functions(), nc = C.n_functions(), nd = D.n_functions();
+            tmp.resize(buf.size());
+            to_shell_index(sph_rows(A), na, ncA, 1, (long)ncB * ncC * ncD, buf.data(), tmp.data());
+            to_shell_index(sph_rows(B), nb, ncB, na, (long)ncC * ncD, tmp.data(), buf.data());
+            to_shell_index(sph_rows(C), nc, ncC, (long)na * nb, ncD, buf.data(), tmp.data());
+            to_shell_index(sph_rows(D), nd, ncD, (long)na * nb * nc, 1, tmp.data(), buf.data());
             for (int ia = 0; ia < na; ++ia)
                 for (int ib = 0; ib < nb; ++ib)
                     for (int ic = 0; ic < nc; ++ic)
                         for (int id = 0; id < nd; ++id) {
                             const int i = offset_[sa] + ia, j = offset_[sb] + ib, k = offset_[sc] + ic,
                                       l = offset_[sd] + id;
-                            const double v = buf[ia][ib][ic][id];
+                            const double v = buf[(((size_t)ia * nb + ib) * nc + ic) * nd + id];
                             eri(i, j, k, l) = v;
                             eri(j, i, k, l) = v;
                             eri(i, j, l, k) = v;

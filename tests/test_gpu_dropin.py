@@ -206,3 +206,34 @@ def test_cd_rohf_one_electron_and_closed_shell(m):
     rcfg.max_iterations = 300
     rh = m.CholeskyRHF(mol4, b4, eng4, rcfg, chol4).compute()
     assert abs(ro.energy_total - rh.energy_total) < 1e-9
+
+
+@pytest.mark.parametrize("basis,nvec,slack,E,etol", [("cc-pcvtz", 394, 0, -7.43270280, 6e-9),
+                                                      ("aug-cc-pcvtz", 590, 1, -7.43270609, 3e-8),
+                                                      ("cc-pcvqz", 770, 0, -7.43271913, 6e-9),
+                                                      ("aug-cc-pcvqz", 1175, 8, -7.43271942, 6e-9)])
+def test_li_known_answers_on_the_gpu(m, basis, nvec, slack, E, etol):
+    """The reference's Li doublet runs (hasil/hasil_test_FCI.md): the GPU
+    pivoted CD at 1e-8 (bitwise the host decomposition) and the drop-in
+    CD-UHF on those vectors reproduce the logged vector count and energy
+    (the count of the augmented sets is set by FP64 rounding at their 1e16+
+    diagonal range, hence the slack)."""
+    mol = m.Molecule(0, 2)
+    mol.add_atom(3, 0.0, 0.0, 0.0)
+    b = m.BasisSet(basis, mol, BD)
+    eng = m.IntegralEngine(mol, b)
+    eri = eng.compute_eri()
+    print(f"Li {basis} n={b.n_basis_functions()}", flush=True)
+    c_gpu, c_host = m.CholeskyERI(1e-8), m.CholeskyERI(1e-8)
+    r_gpu, r_host = c_gpu.decompose_gpu(eri), c_host.decompose(eri)
+    assert r_gpu.n_vectors == r_host.n_vectors and abs(r_gpu.n_vectors - nvec) <= slack
+    assert np.array_equal(np.array(c_gpu.get_L_vectors()), np.array(c_host.get_L_vectors()))
+    cfg = m.CholeskyUHFConfig()
+    cfg.print_level = 0
+    cfg.cholesky_threshold = 1e-8
+    cfg.max_iterations = 100
+    uhf = m.CholeskyUHF(mol, b, eng, 2, 1, cfg)
+    uhf.set_cholesky_vectors(c_gpu.get_L_vectors())
+    r = uhf.compute()
+    print(f"  vectors {r_gpu.n_vectors} (log {nvec}), E_UHF {r.energy_total:.9f} (log {E})", flush=True)
+    assert r.converged and abs(r.energy_total - E) <= etol

@@ -1,6 +1,9 @@
-"""The build's own s/p integral engine (the libint2 stand-in for the H-chain
-workload): textbook energies, an independent closed-form/derivative checker,
-and symmetry.  Runs on the CPU."""
+"""The build's own integral engine (McMurchie-Davidson, s to g shells, real
+solid harmonics for d and above; the libint2 stand-in): textbook energies, an
+independent closed-form/derivative checker for s/p, grid quadrature for
+Cartesian d/f/g overlaps, orthonormality and rotation invariance of the
+solid harmonics, and symmetry.  Runs on the CPU.  The Li known answers of the
+reference's own runs are in test_known_answers.py."""
 import math
 import os
 
@@ -148,8 +151,93 @@ def test_eri_symmetry_and_psd(m):
 def test_unsupported_shell_raises(m):
     mol = mol_of(m, [(1, 0, 0, 0)])
     b = m.BasisSet()
-    sh = m.Shell(m.AngularMomentum.D, 0, [0.0, 0.0, 0.0])
+    sh = m.Shell(m.AngularMomentum.H, 0, [0.0, 0.0, 0.0])
     sh.add_primitive(1.0, 1.0)
     b.add_shell(sh)
     with pytest.raises(RuntimeError):
         m.IntegralEngine(mol, b)
+
+
+def _gbs(tmp_path, kind, shells):
+    """A one-element .gbs file (element X at every atom) of single-primitive
+    shells [(letter, exponent)], spherical or cartesian."""
+    lines = [kind, "", "****", "H     0"]
+    for letter, a in shells:
+        lines += [f"{letter}   1   1.00", f"   {a:.10f}   1.0000000"]
+    lines.append("****")
+    f = tmp_path / f"t_{kind}.gbs"
+    f.write_text("\n".join(lines) + "\n")
+    return str(tmp_path), f"t_{kind}"
+
+
+def _cart_list(L):
+    return [(a, b, L - a - b) for a in range(L, -1, -1) for b in range(L - a, -1, -1)]
+
+
+def test_cartesian_overlap_quadrature(m, tmp_path):
+    """Cartesian d, f and g overlaps on two centres against trapezoidal
+    quadrature on a grid (exponents >= 0.8, spacing 0.15: aliasing and
+    truncation below 1e-12 of the largest element)."""
+    shells = [("D", 0.9), ("F", 1.1), ("G", 0.8), ("P", 1.3)]
+    d, name = _gbs(tmp_path, "cartesian", shells)
+    centres = [(0.0, 0.0, 0.0), (0.35, -0.25, 0.6)]
+    mol = mol_of(m, [(1, *c) for c in centres])
+    b = m.BasisSet(name, mol, d)
+    S = m.IntegralEngine(mol, b).compute_overlap()
+    x = np.arange(-9.0, 9.0 + 1e-9, 0.15)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    funcs = []
+    for k in range(b.n_shells()):
+        sh = b.shell(k)
+        L = int(sh.angular_momentum.value)
+        C = np.array(centres[sh.center])
+        (pr,) = sh.primitives
+        dx, dy, dz = X - C[0], Y - C[1], Z - C[2]
+        g = pr.coefficient * np.exp(-pr.exponent * (dx * dx + dy * dy + dz * dz))
+        funcs += [g * dx ** i * dy ** j * dz ** l for i, j, l in _cart_list(L)]
+    assert len(funcs) == b.n_basis_functions() == S.shape[0]
+    h3 = 0.15 ** 3
+    rng = np.random.default_rng(1)
+    n = len(funcs)
+    for mu, nu in {(0, 0), (n - 1, n - 1), (3, n - 2)} | {tuple(rng.integers(0, n, 2)) for _ in range(40)}:
+        q = float(np.sum(funcs[mu] * funcs[nu])) * h3
+        assert abs(S[mu, nu] - q) <= 1e-12 * np.abs(S).max(), (mu, nu, S[mu, nu], q)
+
+
+@pytest.mark.parametrize("letter,L", [("D", 2), ("F", 3), ("G", 4)])
+def test_solid_harmonics_orthonormal(m, tmp_path, letter, L):
+    """One primitive of exponent a: the 2L+1 real solid harmonics are
+    orthogonal, each with the norm of x^L exp(-a r^2) (the shells'
+    coefficient convention), (2L-1)!! (pi/2a)^(3/2) / (4a)^L."""
+    a = 0.7
+    d, name = _gbs(tmp_path, "spherical", [(letter, a)])
+    mol = mol_of(m, [(1, 0.0, 0.0, 0.0)])
+    b = m.BasisSet(name, mol, d)
+    S = m.IntegralEngine(mol, b).compute_overlap()
+    (pr,) = b.shell(0).primitives
+    df = float(np.prod(np.arange(2 * L - 1, 0, -2)))
+    s0 = pr.coefficient ** 2 * df * (math.pi / (2 * a)) ** 1.5 / (4 * a) ** L
+    assert S.shape == (2 * L + 1, 2 * L + 1)
+    assert np.abs(S - s0 * np.eye(2 * L + 1)).max() < 1e-13 * s0
+
+
+def test_rotation_invariance(m, tmp_path):
+    """s..g shells on three centres: rotating the geometry rotates each
+    shell's functions among themselves (an orthogonal map for p and for the
+    real solid harmonics), so the spectra of S, T, V and of the ERI matrix
+    are unchanged."""
+    shells = [("S", 1.2), ("P", 0.9), ("D", 0.8), ("F", 1.0), ("G", 1.1)]
+    d, name = _gbs(tmp_path, "spherical", shells)
+    base = np.array([[0.0, 0.0, 0.0], [0.4, -0.3, 1.1], [-0.7, 0.5, 0.2]])
+    R = sl.expm(np.array([[0, 0.3, -0.8], [-0.3, 0, 0.5], [0.8, -0.5, 0]]))  # a rotation
+    out = []
+    for geo in (base, base @ R.T + np.array([0.2, -0.1, 0.3])):
+        mol = mol_of(m, [(1, *p) for p in geo])
+        b = m.BasisSet(name, mol, d)
+        eng = m.IntegralEngine(mol, b)
+        n = b.n_basis_functions()
+        eri = eng.compute_eri().numpy().reshape(n * n, n * n)
+        out.append([np.linalg.eigvalsh(x) for x in (eng.compute_overlap(), eng.compute_kinetic(),
+                                                   eng.compute_nuclear(), eri)])
+    for w0, w1 in zip(*out):
+        assert np.abs(w0 - w1).max() < 1e-11 * np.abs(w0).max()

@@ -2,7 +2,9 @@
 runs of `python test/test_cholesky_sa_caspt3.py` logged in
 hasil/hasil_test_FCI.md (lines 7-45, 130-168, 253-291, 379-414): the
 basis size, the pivoted CD at 1e-8 of the full ERI matrix (initial maximum
-diagonal, vector count) and the CD-UHF energy on those vectors.  Here the
+diagonal, vector count) and the CD-UHF energy on those vectors; and the CD
+of the cc-pVTZ / cc-pVQZ Li runs in include/mshqc/mcscf/hasilterbaiksaatini.md
+(lines 92-121, 179-208), which log no energy.  Here the
 build's integral engine and host CholeskyERI::decompose feed the oracle's
 CD-UHF restatement (the GPU chain through the drop-in classes is in
 test_gpu_dropin.py).  Runs on the CPU.
@@ -61,3 +63,15 @@ def test_li_cd_uhf(m, basis, N, nvec, slack, maxdiag, dtol, E, etol):
     H = eng.compute_kinetic() + eng.compute_nuclear()
     u = scf_ref.cd_uhf(eng.compute_overlap(), H, L, 2, 1, mol.nuclear_repulsion_energy(), max_iterations=100)
     assert abs(u["energy"] - E) <= etol, (u["energy"], E)
+
+
+@pytest.mark.parametrize("basis,N,nvec,maxdiag,dtol", [("cc-pvtz", 30, 316, 8.827692e6, 1e-7),
+                                                     ("cc-pvqz", 55, 650, 1.60491e7, 5e-6)])
+def test_li_cd_counts(m, basis, N, nvec, maxdiag, dtol):
+    mol, b, eng = li(m, basis)
+    assert b.n_basis_functions() == N
+    eri = eng.compute_eri()
+    d = np.einsum("ijij->ij", eri.numpy()).max()
+    assert abs(d - maxdiag) <= dtol * maxdiag
+    r = m.CholeskyERI(1e-8).decompose(eri)
+    assert r.converged and r.n_vectors == nvec

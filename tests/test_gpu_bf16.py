@@ -109,3 +109,52 @@ def test_ump2_energy_slices(ctx, k):
     print(f"ump2 k={k}: {E!r} oracle {E_ref!r}", flush=True)
     for x, y in zip(E, E_ref):
         assert abs(x - y) <= UMP2_TOL[k] * abs(y)
+
+
+def test_precision_workload_full_size(ctx):
+    """The o=100, v=1100, Q=4000 precision workload at full size: the FP64
+    path satisfies the size-independent identities <B, dE/dB> = 4E,
+    sum dE/deps = 0 and sum eps dE/deps = -E (1e-12 relative), and the
+    split-BF16 form (5 slices) stays within the stated tolerance of it:
+    |dE| <= 1e-6 Eh, and dE/dB and dE/deps each within 1e-6 relative (norm)."""
+    import torch
+
+    B, eo, ev = to_dev(*syn.workload("mp2_o100"))
+    print(f"precision workload Q={B.shape[0]} o={B.shape[1]} v={B.shape[2]}", flush=True)
+    E, g, d_o, d_v = ctx.rmp2_grad(B, eo, ev)
+    assert abs((B * g).sum().item() - 4 * E) <= 1e-12 * abs(E)
+    assert abs(d_o.sum().item() + d_v.sum().item()) <= 1e-12 * abs(E)
+    assert abs((eo * d_o).sum().item() + (ev * d_v).sum().item() + E) <= 1e-12 * abs(E)
+    Eb, gb, d_ob, d_vb = ctx.rmp2_grad(B, eo, ev, precision=5)
+    rel_B = (torch.linalg.norm(gb - g) / torch.linalg.norm(g)).item()
+    d, db = torch.cat([d_o, d_v]), torch.cat([d_ob, d_vb])
+    rel_e = (torch.linalg.norm(db - d) / torch.linalg.norm(d)).item()
+    print(f"  E {E!r} bf16x5 dE {Eb - E:.3e} dE/dB rel {rel_B:.3e} dE/deps rel {rel_e:.3e}", flush=True)
+    assert abs(Eb - E) <= 1e-6
+    assert rel_B <= 1e-6 and rel_e <= 1e-6
+
+
+def test_ump2_workload_full_q_v(ctx):
+    """The CD-UMP2 workload's full Q = 5600 and v (1380 / 1382) with a tenth
+    of its occupied orbitals (12 / 11): the FP64 path's identities
+    (<B, dE/dB> = 4E over both spins, sum dE/deps = 0, sum eps dE/deps = -E)
+    and the split-BF16 form within 1e-6 Eh and 1e-6 relative per gradient."""
+    import torch
+
+    (Ba, eoa, eva), (Bb, eob, evb) = syn.ump2_inputs(5600, 12, 1380, 11, 1382, 1005)
+    Ba, eoa, eva, Bb, eob, evb = to_dev(Ba, eoa, eva, Bb, eob, evb)
+    print(f"ump2 Q=5600 oa=12 va=1380 ob=11 vb=1382", flush=True)
+    E, (dBa, dBb, doa, dva, dob, dvb) = ctx.ump2_grad(Ba, Bb, eoa, eva, eob, evb)
+    Et = sum(E)
+    assert abs((Ba * dBa).sum().item() + (Bb * dBb).sum().item() - 4 * Et) <= 1e-12 * abs(Et)
+    ds = [doa, dva, dob, dvb]
+    assert abs(sum(x.sum().item() for x in ds)) <= 1e-12 * abs(Et)
+    eps = [eoa, eva, eob, evb]
+    assert abs(sum((e * x).sum().item() for e, x in zip(eps, ds)) + Et) <= 1e-12 * abs(Et)
+    Eb, (gBa, gBb, *gd) = ctx.ump2_grad(Ba, Bb, eoa, eva, eob, evb, precision=5)
+    for got, want in zip(Eb, E):
+        assert abs(got - want) <= 1e-6
+    for got, want in ((gBa, dBa), (gBb, dBb), (torch.cat(gd), torch.cat(ds))):
+        rel = (torch.linalg.norm(got - want) / torch.linalg.norm(want)).item()
+        print(f"  gradient rel {rel:.3e}", flush=True)
+        assert rel <= 1e-6

@@ -7,20 +7,21 @@
 
 namespace {
 
-constexpr int TM = 64, TN = 64, KC = 16, NT = 256;
+constexpr int TM = 64, TN = 64, KC = 32, NT = 256;
+constexpr int PR = TM * KC / NT;  // panel elements per thread and step
 
 // Panel loader: S[kk][c] = M_b(row0 + c, k0 + kk) for a matrix whose element
 // (r, k) sits at base[r * ld + k] (KCONTIG) or base[k * ld + r] (!KCONTIG).
 template <bool KCONTIG>
 struct Panel {
     static __device__ __forceinline__ void load(const double* __restrict__ base, long ld, int rows,
-                                                int kmax, int row0, int k0, int tid, double (&reg)[4]) {
+                                                int kmax, int row0, int k0, int tid, double (&reg)[PR]) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < PR; ++r) {
             int kk, c;
             if (KCONTIG) {
-                kk = tid & 15;
-                c = (tid >> 4) + 16 * r;
+                kk = tid % KC;
+                c = tid / KC + (NT / KC) * r;
             } else {
                 c = tid & 63;
                 kk = (tid >> 6) + 4 * r;
@@ -32,13 +33,13 @@ struct Panel {
             reg[r] = x;
         }
     }
-    static __device__ __forceinline__ void store(double (*S)[TM + 1], int tid, const double (&reg)[4]) {
+    static __device__ __forceinline__ void store(double (*S)[TM + 1], int tid, const double (&reg)[PR]) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < PR; ++r) {
             int kk, c;
             if (KCONTIG) {
-                kk = tid & 15;
-                c = (tid >> 4) + 16 * r;
+                kk = tid % KC;
+                c = tid / KC + (NT / KC) * r;
             } else {
                 c = tid & 63;
                 kk = (tid >> 6) + 4 * r;
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmF64 g, double* __restr
 
     const int nk = (g.K + KC - 1) / KC;
     const long total = (long)(b_hi - b_lo) * nk;
-    double ra[4], rx[4];
+    double ra[PR], rx[PR];
     auto fetch = [&](long step) {
         const int b = b_lo + (int)(step / nk);
         const int k0 = (int)(step % nk) * KC;

@@ -13,6 +13,7 @@
 #include <cmath>
 
 #include "common.h"
+#include "mfma_f64.h"
 
 namespace {
 
@@ -57,6 +58,128 @@ __global__ __launch_bounds__(AMP_THREADS) void omp2_amp_kernel(
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = e;
     __syncthreads();
     if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// Small symmetric products for the G_oo contractions (o x o outputs, o <= 64):
+// part[wg][m][n] = sum over workgroup wg's batch entries b of
+// sum_k A_b[m][k] A_b[n][k], A_b[m][k] = A[bo sA + bi sA2 + m lda + k]
+// (b = bo inner + bi).  Each wave keeps the whole G x G grid of 16 x 16
+// FP64 MFMA tiles and walks every fourth entry; one loaded fragment serves
+// as both MFMA operands.  The four waves are summed through LDS in wave
+// order, so the partials are reproducible.
+constexpr int SY_THREADS = 256;
+
+template <int G>
+__global__ __launch_bounds__(SY_THREADS) void omp2_syrk_small_kernel(const double* __restrict__ A, long lda,
+                                                                     long sA, long sA2, int inner, int M,
+                                                                     int K, int batch, int bper,
+                                                                     double* __restrict__ part) {
+    __shared__ double red[64 * 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, kq = lane >> 4;
+    v4d acc[G][G];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int h = 0; h < G; ++h) acc[g][h] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int b_lo = blockIdx.x * bper, b_hi = min(batch, b_lo + bper);
+    for (int b = b_lo + wave; b < b_hi; b += 4) {
+        const int bo = b / inner, bi = b - bo * inner;
+        const double* Ab = A + (long)bo * sA + (long)bi * sA2;
+        // the next step's fragments load while this step's MFMAs run
+        auto fetch = [&](int k0, double(&f)[G]) {
+            const int k = k0 + kq;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int m = 16 * g + r;
+                f[g] = (m < M && k < K) ? Ab[(long)m * lda + k] : 0.0;
+            }
+        };
+        double f[G], fn[G];
+        fetch(0, f);
+        for (int k0 = 0; k0 < K; k0 += 4) {
+            fetch(k0 + 4, fn);
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int h = 0; h < G; ++h) acc[g][h] = mfma_f64(f[g], f[h], acc[g][h]);
+#pragma unroll
+            for (int g = 0; g < G; ++g) f[g] = fn[g];
+        }
+    }
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w) {
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int h = 0; h < G; ++h)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int idx = (16 * g + mfma_row(lane, q)) * 64 + 16 * h + mfma_col(lane);
+                        red[idx] = w == 0 ? acc[g][h][q] : red[idx] + acc[g][h][q];
+                    }
+        }
+        __syncthreads();
+    }
+    double* out = part + (long)blockIdx.x * M * M;
+    for (int e = threadIdx.x; e < M * M; e += SY_THREADS) out[e] = red[(e / M) * 64 + e % M];
+}
+
+// C[m][n] (+)= alpha * sum_z part[z][m][n], z in order.
+__global__ void omp2_sum_parts_kernel(const double* __restrict__ part, int nz, int M, double* __restrict__ C,
+                                      long ldc, double alpha) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= M * M) return;
+    double s = 0.0;
+    for (int z = 0; z < nz; ++z) s += part[(long)z * M * M + e];
+    C[(long)(e / M) * ldc + e % M] += alpha * s;
+}
+
+// C (M x M, accumulated) += alpha * sum_b A_b A_b^T for M <= 64 (else the
+// generic batch-reduced GEMM).
+int small_syrk(mshqc_hip_ctx* ctx, const double* A, long lda, long sA, long sA2, int inner, int M, int K,
+               int batch, double alpha, double* C, long ldc) {
+    if (M <= 0 || batch <= 0) return MSHQC_HIP_OK;
+    if (M > 64) {
+        GemmF64 g;
+        g.A = A;
+        g.lda = lda;
+        g.sA = sA;
+        g.sA2 = sA2;
+        g.X = A;
+        g.ldx = lda;
+        g.sX = sA;
+        g.sX2 = sA2;
+        g.inner = inner;
+        g.transX = true;
+        g.C = C;
+        g.ldc = ldc;
+        g.M = g.N = M;
+        g.K = K;
+        g.batch = batch;
+        g.reduce_batch = true;
+        g.splits = std::min<long>(batch, std::max<long>(1, (1L << 25) / ((long)M * M)));
+        g.splits = std::min(g.splits, 1024);
+        g.alpha = alpha;
+        g.accumulate = true;
+        return gemm_f64(ctx, g);
+    }
+    const int nwg = std::min(1024, (batch + 3) / 4);
+    const int bper = (batch + nwg - 1) / nwg;
+    const int nz = (batch + bper - 1) / bper;
+    double* part = static_cast<double*>(workspace(ctx, WS_TMP2, sizeof(double) * (size_t)nz * M * M));
+    if (!part) return MSHQC_HIP_ERR_ALLOC;
+    const int G = (M + 15) / 16;
+    switch (G) {
+        case 1: omp2_syrk_small_kernel<1><<<nz, SY_THREADS, 0, ctx->stream>>>(A, lda, sA, sA2, inner, M, K, batch, bper, part); break;
+        case 2: omp2_syrk_small_kernel<2><<<nz, SY_THREADS, 0, ctx->stream>>>(A, lda, sA, sA2, inner, M, K, batch, bper, part); break;
+        case 3: omp2_syrk_small_kernel<3><<<nz, SY_THREADS, 0, ctx->stream>>>(A, lda, sA, sA2, inner, M, K, batch, bper, part); break;
+        default: omp2_syrk_small_kernel<4><<<nz, SY_THREADS, 0, ctx->stream>>>(A, lda, sA, sA2, inner, M, K, batch, bper, part); break;
+    }
+    LAUNCH_CHECK(ctx);
+    omp2_sum_parts_kernel<<<(M * M + 255) / 256, 256, 0, ctx->stream>>>(part, nz, M, C, ldc, alpha);
+    LAUNCH_CHECK(ctx);
+    return MSHQC_HIP_OK;
 }
 
 struct SpinPair {
@@ -123,27 +246,7 @@ int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int k
         if (w.e_slot && (rc = reduce_sum(ctx, part, groups, w.e_slot, true))) return rc;
         const long vv = (long)s.vX * s.vY;
         if (w.GooX) {  // sum_k T_k T_k^T, T_k = [oX][vX vY], cut into one chunk per a
-            GemmF64 g;
-            g.A = T;
-            g.lda = vv;
-            g.sA = per_k;
-            g.sA2 = s.vY;
-            g.X = T;
-            g.ldx = vv;
-            g.sX = per_k;
-            g.sX2 = s.vY;
-            g.inner = s.vX;
-            g.transX = true;
-            g.C = w.GooX;
-            g.ldc = s.oX;
-            g.M = g.N = s.oX;
-            g.K = s.vY;
-            g.batch = nk * s.vX;
-            g.reduce_batch = true;
-            g.splits = red_splits(g.batch, s.oX, s.oX);
-            g.alpha = so;
-            g.accumulate = true;
-            { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
+            { ProfScope ps(ctx, PROF_GEMM); rc = small_syrk(ctx, T, vv, per_k, s.vY, s.vX, s.oX, s.vY, nk * s.vX, so, w.GooX, s.oX); }
             if (rc) return rc;
         }
         if (w.GvvX) {  // sum_{k,i} T_ki T_ki^T, T_ki = [vX][vY]
@@ -189,27 +292,7 @@ int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int k
             if (rc) return rc;
         }
         if (w.GooY) {  // one block holds every k: -sum_{i,a} T_(.ia) T_(.ia)^T over b
-            GemmF64 g;
-            g.A = T;
-            g.lda = per_k;
-            g.sA = vv;
-            g.sA2 = s.vY;
-            g.X = T;
-            g.ldx = per_k;
-            g.sX = vv;
-            g.sX2 = s.vY;
-            g.inner = s.vX;
-            g.transX = true;
-            g.C = w.GooY;
-            g.ldc = s.oY;
-            g.M = g.N = nk;
-            g.K = s.vY;
-            g.batch = s.oX * s.vX;
-            g.reduce_batch = true;
-            g.splits = red_splits(g.batch, nk, nk);
-            g.alpha = -1.0;
-            g.accumulate = true;
-            { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
+            { ProfScope ps(ctx, PROF_GEMM); rc = small_syrk(ctx, T, per_k, vv, s.vY, s.vX, nk, s.vY, s.oX * s.vX, -1.0, w.GooY, s.oY); }
             if (rc) return rc;
         }
     }

@@ -26,6 +26,10 @@ def _spin(rng, Q, o, v):
     (17, 3, 9, 0, 12, None, 3),     # no beta electrons
     (19, 1, 7, 2, 6, None, 4),      # one alpha electron: no alpha pairs
     (40, 7, 30, 5, 32, 13440, 5),   # blocks of 2 occupied indices, spin-swapped pass for G_oo^b
+    (24, 20, 9, 18, 11, None, 6),   # G_oo in 2 x 2 MFMA tiles
+    (20, 40, 8, 38, 10, None, 7),   # 3 x 3 tiles
+    (16, 60, 6, 52, 14, None, 8),   # 4 x 4 tiles
+    (12, 70, 5, 66, 9, None, 9),    # o > 64: the generic batch-reduced GEMM
 ])
 def test_omp2_opdm_matches_oracle(ctx, Q, oa, va, ob, vb, zblock, seed):
     rng = np.random.default_rng(seed)

@@ -182,6 +182,31 @@ int small_syrk(mshqc_hip_ctx* ctx, const double* A, long lda, long sA, long sA2,
     return MSHQC_HIP_OK;
 }
 
+// Upper blocks of a symmetric n x n matrix (row stride ld) from the lower
+// ones: tile (ti, tj) of SYM_TILE x SYM_TILE lies in the block pair
+// (ti / tpb, tj / tpb); where the column block exceeds the row block it is
+// overwritten by the transpose of tile (tj, ti), through LDS.
+constexpr int SYM_TILE = 64;
+
+__global__ __launch_bounds__(SYM_TILE * 8) void omp2_mirror_kernel(double* __restrict__ M, long ld, long n,
+                                                                    long tpb) {
+    const long ti = blockIdx.y, tj = blockIdx.x;
+    if (tj / tpb <= ti / tpb) return;
+    __shared__ double t[SYM_TILE][SYM_TILE + 1];
+    const int x = threadIdx.x, y0 = threadIdx.y;
+    // read tile (tj, ti): rows tj*T + y, columns ti*T + x
+    for (int y = y0; y < SYM_TILE; y += 8) {
+        const long r = tj * SYM_TILE + y, c = ti * SYM_TILE + x;
+        if (r < n && c < n) t[y][x] = M[r * ld + c];
+    }
+    __syncthreads();
+    // write tile (ti, tj): element (ti*T + y, tj*T + x) = (tj*T + x, ti*T + y)
+    for (int y = y0; y < SYM_TILE; y += 8) {
+        const long r = ti * SYM_TILE + y, c = tj * SYM_TILE + x;
+        if (r < n && c < n) M[r * ld + c] = t[x][y];
+    }
+}
+
 struct SpinPair {
     const double* BX;  // [Q][oX][vX]
     const double* BY;  // [Q][oY][vY]
@@ -223,20 +248,40 @@ int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int k
     for (int k0 = 0; k0 < s.oY; k0 += kb) {
         const int nk = std::min(kb, s.oY - k0);
         const long ldv = (long)nk * s.vY;
-        // V = B_X^T B_Y[:, k0:k0+nk, :] over Q
-        GemmF64 gv;
-        gv.A = s.BX;
-        gv.lda = (long)s.oX * s.vX;
-        gv.transA = true;
-        gv.X = s.BY + (long)k0 * s.vY;
-        gv.ldx = (long)s.oY * s.vY;
-        gv.C = V;
-        gv.ldc = ldv;
-        gv.M = s.oX * s.vX;
-        gv.N = (int)ldv;
-        gv.K = Q;
-        { ProfScope ps(ctx, PROF_PAIR); rc = gemm_f64(ctx, gv); }
-        if (rc) return rc;
+        // V = B_X^T B_Y[:, k0:k0+nk, :] over Q, as blocks of rows r0.. and
+        // columns c0..
+        auto v_block = [&](long r0, long rn, long c0, long cn) {
+            GemmF64 gv;
+            gv.A = s.BX + r0;
+            gv.lda = (long)s.oX * s.vX;
+            gv.transA = true;
+            gv.X = s.BY + (long)k0 * s.vY + c0;
+            gv.ldx = (long)s.oY * s.vY;
+            gv.C = V + r0 * ldv + c0;
+            gv.ldc = ldv;
+            gv.M = (int)rn;
+            gv.N = (int)cn;
+            gv.K = Q;
+            ProfScope ps(ctx, PROF_PAIR);
+            return gemm_f64(ctx, gv);
+        };
+        const long nov = (long)s.oX * s.vX;
+        if (s.same && nk == s.oY) {
+            // Same spin with every k in one block: V is symmetric.  The
+            // blocks on and below the diagonal are formed (each element the
+            // same ordered sum over Q as in the full product) and the rest
+            // mirrored.
+            const long rb = (nov + 4 * SYM_TILE - 1) / (4 * SYM_TILE) * SYM_TILE;
+            for (long r0 = 0; r0 < nov; r0 += rb)
+                for (long c0 = 0; c0 <= r0; c0 += rb)
+                    if ((rc = v_block(r0, std::min(rb, nov - r0), c0, std::min(rb, nov - c0)))) return rc;
+            const long nt = (nov + SYM_TILE - 1) / SYM_TILE;
+            omp2_mirror_kernel<<<dim3((unsigned)nt, (unsigned)nt), dim3(SYM_TILE, 8), 0, ctx->stream>>>(
+                V, ldv, nov, rb / SYM_TILE);
+            LAUNCH_CHECK(ctx);
+        } else if ((rc = v_block(0, nov, 0, ldv))) {
+            return rc;
+        }
         const long total = (long)nk * per_k;
         const int groups = (int)std::min<long>(AMP_MAX_GROUPS, (total + AMP_THREADS - 1) / AMP_THREADS);
         omp2_amp_kernel<<<groups, AMP_THREADS, 0, ctx->stream>>>(V, ldv, s.oX, s.vX, s.vY, k0, nk, s.eoX,

@@ -25,10 +25,12 @@ constexpr int AMP_MAX_GROUPS = 4096;
 // a == b (cholesky_omp2.cc:115-137); else t = (ia|kb) / D (:161-180).
 // |D| < 1e-12 leaves t = 0.  part[g] = sum of t^2 D over workgroup g's
 // elements (fixed grid-stride assignment, so the sum is reproducible).
+// full != 0: V is the whole symmetric (ia|kb) matrix (same spin, one block),
+// so (ib|ka) is read as its mirror (ka|ib), contiguous along b.
 __global__ __launch_bounds__(AMP_THREADS) void omp2_amp_kernel(
     const double* __restrict__ V, long ldv, int oX, int vX, int vY, int k0, int nk,
     const double* __restrict__ eoX, const double* __restrict__ evX, const double* __restrict__ eoY,
-    const double* __restrict__ evY, int same, double* __restrict__ T, double* __restrict__ part) {
+    const double* __restrict__ evY, int same, int full, double* __restrict__ T, double* __restrict__ part) {
     const long total = (long)nk * oX * vX * vY;
     double e = 0.0;
     for (long t = (long)blockIdx.x * AMP_THREADS + threadIdx.x; t < total;
@@ -47,7 +49,9 @@ __global__ __launch_bounds__(AMP_THREADS) void omp2_amp_kernel(
             if (!same) {
                 val = g / D;
             } else if (i != k && a != b) {
-                val = (g - V[((long)i * vX + b) * ldv + (long)kk * vY + a]) / D;
+                const double gx = full ? V[((long)k * vX + a) * ldv + (long)i * vY + b]
+                                       : V[((long)i * vX + b) * ldv + (long)kk * vY + a];
+                val = (g - gx) / D;
             }
         }
         T[t] = val;
@@ -285,8 +289,8 @@ int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int k
         const long total = (long)nk * per_k;
         const int groups = (int)std::min<long>(AMP_MAX_GROUPS, (total + AMP_THREADS - 1) / AMP_THREADS);
         omp2_amp_kernel<<<groups, AMP_THREADS, 0, ctx->stream>>>(V, ldv, s.oX, s.vX, s.vY, k0, nk, s.eoX,
-                                                                 s.evX, s.eoY, s.evY, s.same ? 1 : 0, T,
-                                                                 part);
+                                                                 s.evX, s.eoY, s.evY, s.same ? 1 : 0,
+                                                                 (s.same && nk == s.oY) ? 1 : 0, T, part);
         LAUNCH_CHECK(ctx);
         if (w.e_slot && (rc = reduce_sum(ctx, part, groups, w.e_slot, true))) return rc;
         const long vv = (long)s.vX * s.vY;

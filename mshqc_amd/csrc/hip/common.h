@@ -135,8 +135,17 @@ struct GemmF64 {
     int splits = 1;
     int inner = 1;
     long sA2 = 0, sX2 = 0;
+    // M == N and C symmetric: only the 64 x 64 tiles on and below the
+    // diagonal are formed (mirror_upper then fills the rest)
+    bool lower_only = false;
 };
 int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g);
+
+// Square n x n matrix (row stride ld): the elements of every tile
+// (64 * tpb rows and columns per block) whose column block exceeds its row
+// block are overwritten by their transposes, so a lower-block-triangle
+// product becomes the whole symmetric matrix.
+int mirror_upper(mshqc_hip_ctx* ctx, double* M, long ld, long n, long tpb);
 
 // Deterministic single-workgroup sum of x[0..n) into *out (device), using a
 // fixed per-thread stride and a fixed tree; Neumaier compensation per thread.

@@ -57,6 +57,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmF64 g, double* __restr
     __shared__ double Xs[KC][TN + 1];
     int mt, ntile;
     if (!grouped_tile((g.M + TM - 1) / TM, (g.N + TN - 1) / TN, mt, ntile)) return;
+    if (g.lower_only && ntile > mt) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int m0 = mt * TM, n0 = ntile * TN;
@@ -139,10 +140,12 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmF64 g, double* __restr
 }
 
 __global__ void split_reduce_kernel(const double* __restrict__ part, int splits, int M, int N,
-                                    double* __restrict__ C, long ldc, double alpha, int accumulate) {
+                                    double* __restrict__ C, long ldc, double alpha, int accumulate,
+                                    int lower_only) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)M * N) return;
     const int m = (int)(idx / N), n = (int)(idx % N);
+    if (lower_only && n / TN > m / TM) return;  // a tile no split formed
     double s = 0.0;
     for (int z = 0; z < splits; ++z) s += part[(long)z * M * N + idx];
     double* p = C + (long)m * ldc + n;
@@ -201,9 +204,40 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
     if (part) {
         const long tot = (long)g.M * g.N;
         split_reduce_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(
-            part, splits, g.M, g.N, g.C, g.ldc, g.alpha, g.accumulate ? 1 : 0);
+            part, splits, g.M, g.N, g.C, g.ldc, g.alpha, g.accumulate ? 1 : 0, g.lower_only ? 1 : 0);
         LAUNCH_CHECK(ctx);
     }
+    return MSHQC_HIP_OK;
+}
+
+// Tile (ti, tj) of 64 x 64 lies in the block pair (ti / tpb, tj / tpb);
+// where the column block exceeds the row block it is overwritten by the
+// transpose of tile (tj, ti), through LDS.
+constexpr int MIR_TILE = 64;
+
+static __global__ __launch_bounds__(MIR_TILE * 8) void mirror_upper_kernel(double* __restrict__ M, long ld,
+                                                                            long n, long tpb) {
+    const long ti = blockIdx.y, tj = blockIdx.x;
+    if (tj / tpb <= ti / tpb) return;
+    __shared__ double t[MIR_TILE][MIR_TILE + 1];
+    const int x = threadIdx.x, y0 = threadIdx.y;
+    for (int y = y0; y < MIR_TILE; y += 8) {  // tile (tj, ti)
+        const long r = tj * MIR_TILE + y, c = ti * MIR_TILE + x;
+        if (r < n && c < n) t[y][x] = M[r * ld + c];
+    }
+    __syncthreads();
+    for (int y = y0; y < MIR_TILE; y += 8) {  // element (ti T + y, tj T + x) = (tj T + x, ti T + y)
+        const long r = ti * MIR_TILE + y, c = tj * MIR_TILE + x;
+        if (r < n && c < n) M[r * ld + c] = t[x][y];
+    }
+}
+
+int mirror_upper(mshqc_hip_ctx* ctx, double* M, long ld, long n, long tpb) {
+    if (n <= 0) return MSHQC_HIP_OK;
+    const long nt = (n + MIR_TILE - 1) / MIR_TILE;
+    if (nt > 65535) return set_error(ctx, MSHQC_HIP_ERR_ARG, "mirror_upper: matrix too large");
+    mirror_upper_kernel<<<dim3((unsigned)nt, (unsigned)nt), dim3(MIR_TILE, 8), 0, ctx->stream>>>(M, ld, n, tpb);
+    LAUNCH_CHECK(ctx);
     return MSHQC_HIP_OK;
 }
 

@@ -105,8 +105,9 @@ int q_chunk(int Q, long per_q, size_t cap) {
 
 constexpr size_t kTmpCap = size_t(1) << 28;  // doubles (2 GiB) per temporary
 
-int gemm_splits(int M, int N, int batch) {
-    const long tiles = (long)((M + 63) / 64) * ((N + 63) / 64);
+int gemm_splits(int M, int N, int batch, bool lower_only = false) {
+    const long tm = (M + 63) / 64, tn = (N + 63) / 64;
+    const long tiles = lower_only ? tm * (tm + 1) / 2 : tm * tn;  // tiles that do work
     long s = (2048 + tiles - 1) / tiles;
     return (int)std::max(1L, std::min((long)batch, s));
 }
@@ -176,11 +177,13 @@ int mshqc_hip_jk(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double
             g2.M = g2.N = g2.K = n;
             g2.batch = nq;
             g2.reduce_batch = true;
-            g2.splits = gemm_splits(n, n, nq);
+            g2.splits = gemm_splits(n, n, nq, true);
             g2.accumulate = q0 > 0;
+            g2.lower_only = true;  // K_s = sum_q L^q P_s L^q is symmetric
             { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
             if (rc) return rc;
         }
+        if ((rc = mirror_upper(ctx, Ks, n, n, 1))) return rc;
     }
     hipEventRecord(ctx->ev1, ctx->stream);
     if ((rc = ctx_finish(ctx))) return rc;
@@ -250,11 +253,13 @@ int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const
         g2.K = m;
         g2.batch = nq;
         g2.reduce_batch = true;
-        g2.splits = gemm_splits(n, n, nq);
+        g2.splits = gemm_splits(n, n, nq, true);
         g2.accumulate = q0 > 0;
+        g2.lower_only = true;  // symmetric
         { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
             if (rc) return rc;
     }
+    if ((rc = mirror_upper(ctx, K, n, n, 1))) return rc;
     hipEventRecord(ctx->ev1, ctx->stream);
     if ((rc = ctx_finish(ctx))) return rc;
     hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
@@ -343,13 +348,16 @@ int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const d
                 g2.K = ms;
                 g2.batch = nq;
                 g2.reduce_batch = true;
-                g2.splits = gemm_splits(n, n, nq);
+                g2.splits = gemm_splits(n, n, nq, true);
                 g2.accumulate = q0 > 0;
+                g2.lower_only = true;  // symmetric
                 { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
                 if (rc) return rc;
             }
         }
     }
+    if (na > 0 && (rc = mirror_upper(ctx, Ka, n, n, 1))) return rc;
+    if (nb > 0 && (rc = mirror_upper(ctx, Kb, n, n, 1))) return rc;
     hipEventRecord(ctx->ev1, ctx->stream);
     if ((rc = ctx_finish(ctx))) return rc;
     hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);

@@ -186,30 +186,7 @@ int small_syrk(mshqc_hip_ctx* ctx, const double* A, long lda, long sA, long sA2,
     return MSHQC_HIP_OK;
 }
 
-// Upper blocks of a symmetric n x n matrix (row stride ld) from the lower
-// ones: tile (ti, tj) of SYM_TILE x SYM_TILE lies in the block pair
-// (ti / tpb, tj / tpb); where the column block exceeds the row block it is
-// overwritten by the transpose of tile (tj, ti), through LDS.
-constexpr int SYM_TILE = 64;
-
-__global__ __launch_bounds__(SYM_TILE * 8) void omp2_mirror_kernel(double* __restrict__ M, long ld, long n,
-                                                                    long tpb) {
-    const long ti = blockIdx.y, tj = blockIdx.x;
-    if (tj / tpb <= ti / tpb) return;
-    __shared__ double t[SYM_TILE][SYM_TILE + 1];
-    const int x = threadIdx.x, y0 = threadIdx.y;
-    // read tile (tj, ti): rows tj*T + y, columns ti*T + x
-    for (int y = y0; y < SYM_TILE; y += 8) {
-        const long r = tj * SYM_TILE + y, c = ti * SYM_TILE + x;
-        if (r < n && c < n) t[y][x] = M[r * ld + c];
-    }
-    __syncthreads();
-    // write tile (ti, tj): element (ti*T + y, tj*T + x) = (tj*T + x, ti*T + y)
-    for (int y = y0; y < SYM_TILE; y += 8) {
-        const long r = ti * SYM_TILE + y, c = tj * SYM_TILE + x;
-        if (r < n && c < n) M[r * ld + c] = t[x][y];
-    }
-}
+constexpr int SYM_TILE = 64;  // tile of the shared mirror (mirror_upper)
 
 struct SpinPair {
     const double* BX;  // [Q][oX][vX]
@@ -279,10 +256,7 @@ int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int k
             for (long r0 = 0; r0 < nov; r0 += rb)
                 for (long c0 = 0; c0 <= r0; c0 += rb)
                     if ((rc = v_block(r0, std::min(rb, nov - r0), c0, std::min(rb, nov - c0)))) return rc;
-            const long nt = (nov + SYM_TILE - 1) / SYM_TILE;
-            omp2_mirror_kernel<<<dim3((unsigned)nt, (unsigned)nt), dim3(SYM_TILE, 8), 0, ctx->stream>>>(
-                V, ldv, nov, rb / SYM_TILE);
-            LAUNCH_CHECK(ctx);
+            if ((rc = mirror_upper(ctx, V, ldv, nov, rb / SYM_TILE))) return rc;
         } else if ((rc = v_block(0, nov, 0, ldv))) {
             return rc;
         }

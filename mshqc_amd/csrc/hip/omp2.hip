@@ -129,14 +129,22 @@ __global__ __launch_bounds__(SY_THREADS) void omp2_syrk_small_kernel(const doubl
     for (int e = threadIdx.x; e < M * M; e += SY_THREADS) out[e] = red[(e / M) * 64 + e % M];
 }
 
-// C[m][n] (+)= alpha * sum_z part[z][m][n], z in order.
-__global__ void omp2_sum_parts_kernel(const double* __restrict__ part, int nz, int M, double* __restrict__ C,
-                                      long ldc, double alpha) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= M * M) return;
+// C[m][n] (+)= alpha * sum_z part[z][m][n]: one workgroup per element,
+// thread t sums z = t, t + 256, ... and a fixed tree adds the threads, so
+// the order is the same on every call.
+__global__ __launch_bounds__(256) void omp2_sum_parts_kernel(const double* __restrict__ part, int nz, int M,
+                                                             double* __restrict__ C, long ldc, double alpha) {
+    __shared__ double sh[256];
+    const int e = blockIdx.x;
     double s = 0.0;
-    for (int z = 0; z < nz; ++z) s += part[(long)z * M * M + e];
-    C[(long)(e / M) * ldc + e % M] += alpha * s;
+    for (int z = threadIdx.x; z < nz; z += 256) s += part[(long)z * M * M + e];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) C[(long)(e / M) * ldc + e % M] += alpha * sh[0];
 }
 
 // C (M x M, accumulated) += alpha * sum_b A_b A_b^T for M <= 64 (else the
@@ -181,7 +189,7 @@ int small_syrk(mshqc_hip_ctx* ctx, const double* A, long lda, long sA, long sA2,
         default: omp2_syrk_small_kernel<4><<<nz, SY_THREADS, 0, ctx->stream>>>(A, lda, sA, sA2, inner, M, K, batch, bper, part); break;
     }
     LAUNCH_CHECK(ctx);
-    omp2_sum_parts_kernel<<<(M * M + 255) / 256, 256, 0, ctx->stream>>>(part, nz, M, C, ldc, alpha);
+    omp2_sum_parts_kernel<<<M * M, 256, 0, ctx->stream>>>(part, nz, M, C, ldc, alpha);
     LAUNCH_CHECK(ctx);
     return MSHQC_HIP_OK;
 }

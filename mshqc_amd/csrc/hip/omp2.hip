@@ -196,6 +196,23 @@ int small_syrk(mshqc_hip_ctx* ctx, const double* A, long lda, long sA, long sA2,
 
 constexpr int SYM_TILE = 64;  // tile of the shared mirror (mirror_upper)
 
+// P[p] = T[k][i] (blocks of vv doubles) for the pairs i < k of an o-orbital
+// block, p = k (k - 1) / 2 + i.
+__global__ __launch_bounds__(256) void omp2_pack_lower_kernel(const double* __restrict__ T, int o, long vv,
+                                                              double* __restrict__ P) {
+    const long npair = (long)o * (o - 1) / 2;
+    for (long p = blockIdx.x; p < npair; p += gridDim.x) {
+        // k from p: the largest k with k (k - 1) / 2 <= p
+        int k = (int)((1.0 + sqrt(1.0 + 8.0 * (double)p)) / 2.0);
+        while ((long)k * (k - 1) / 2 > p) --k;
+        while ((long)(k + 1) * k / 2 <= p) ++k;
+        const int i = (int)(p - (long)k * (k - 1) / 2);
+        const double* src = T + ((long)k * o + i) * vv;
+        double* dst = P + p * vv;
+        for (long e = threadIdx.x; e < vv; e += 256) dst[e] = src[e];
+    }
+}
+
 struct SpinPair {
     const double* BX;  // [Q][oX][vX]
     const double* BY;  // [Q][oY][vY]
@@ -281,11 +298,26 @@ int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int k
             if (rc) return rc;
         }
         if (w.GvvX) {  // sum_{k,i} T_ki T_ki^T, T_ki = [vX][vY]
+            // Same spin with every k in this block: T_ik = -T_ki and T_kk = 0,
+            // so the sum is twice that over i < k, whose blocks are packed
+            // into V (free once the amplitudes are formed).
+            const bool half = s.same && nk == s.oY;
+            const double* src = T;
+            long nb = (long)nk * s.oX;
+            if (half) {
+                nb = (long)s.oX * (s.oX - 1) / 2;
+                if (nb > 0) {
+                    omp2_pack_lower_kernel<<<(unsigned)std::min<long>(nb, 65535), 256, 0, ctx->stream>>>(
+                        T, s.oX, vv, V);
+                    LAUNCH_CHECK(ctx);
+                }
+                src = V;
+            }
             GemmF64 g;
-            g.A = T;
+            g.A = src;
             g.lda = s.vY;
             g.sA = vv;
-            g.X = T;
+            g.X = src;
             g.ldx = s.vY;
             g.sX = vv;
             g.transX = true;
@@ -293,10 +325,10 @@ int spin_pair(mshqc_hip_ctx* ctx, int Q, const SpinPair& s, const Want& w, int k
             g.ldc = s.vX;
             g.M = g.N = s.vX;
             g.K = s.vY;
-            g.batch = nk * s.oX;
+            g.batch = (int)nb;
             g.reduce_batch = true;
-            g.splits = red_splits(nk * s.oX, s.vX, s.vX);
-            g.alpha = sv;
+            g.splits = red_splits(g.batch, s.vX, s.vX);
+            g.alpha = half ? 2.0 * sv : sv;
             g.accumulate = true;
             { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g); }
             if (rc) return rc;

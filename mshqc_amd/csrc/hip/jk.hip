@@ -25,6 +25,8 @@ __global__ __launch_bounds__(256) void dot_per_q_kernel(const double* __restrict
     const bool aligned = (((uintptr_t)Lq) & 15) == 0 && (W == nullptr || (((uintptr_t)W) & 15) == 0);
     if (aligned) {
         const double2* W2 = reinterpret_cast<const double2*>(W);
+        // unrolled so several loads are in flight; the sum stays in k order
+#pragma unroll 4
         for (long k = threadIdx.x; k < n2; k += 256) {
             const double2 l = L2[k];
             if (W) {
@@ -53,6 +55,8 @@ __global__ __launch_bounds__(256) void j_accumulate_kernel(const double* __restr
     if (k >= nn) return;
     double s0 = 0.0, s1 = 0.0;
     if (k + 1 < nn) {
+        // unrolled so several q's loads are in flight; the sums stay in q order
+#pragma unroll 8
         for (int q = 0; q < Q; ++q) {
             const double g = gam[q];
             const double* p = L + (long)q * nn + k;

@@ -3,7 +3,7 @@
 Workload (default, `mp2_o100`): closed-shell CD-RMP2 energy plus gradient
 (dE/dB^Q_ia and dE/deps_p) at Q=4000, o=100, v=1100 on seeded synthetic
 B^Q_ia and orbital energies, inputs resident in HBM before timing starts.
-One step = one full E+grad evaluation.  The split-BF16 form (5 slices) is
+One step = one full E+grad evaluation.  The split-BF16 form (4 slices) is
 timed and reported; the FP64 form is timed beside it as the comparison.
 
 Other workloads (`--workload`):
@@ -481,7 +481,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="bf16x5", help="fp64 | bf16xK (K slices)")
+    ap.add_argument("--precision", default="bf16x4", help="fp64 | bf16xK (K slices)")
     ap.add_argument("--compare", default="fp64", help="second precision timed beside it ('' for none)")
     ap.add_argument("--workload", default="mp2_o100")
     ap.add_argument("--cpu-pairs", type=int, default=16)

@@ -199,6 +199,10 @@ int mshqc_hip_rmp2_t2(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v,
  * compute_ss_beta and compute_os (src/mp2/cholesky_ump2.cc:299-465):
  *   E[0] = 1/4 sum_{ijab in aa} (g_ijab - g_ijba)^2 / D
  *   E[1] = the same for beta,  E[2] = sum_{i in a, j in b, ab} g^2 / D
+ * A spin may have no occupied or no virtual orbitals (o = 0 or v = 0, e.g.
+ * the beta spin of the H atom); its blocks then contribute 0, as the
+ * reference's loops over empty ranges do, and its (empty) arrays may be
+ * NULL.
  */
 int mshqc_hip_ump2_energy(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
                           int Q, int oa, int va, int ob, int vb,

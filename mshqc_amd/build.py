@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import glob
 import os
+import re
 import subprocess
 import sys
 import sysconfig
@@ -35,6 +36,29 @@ def _run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
+# Sources whose kernels stream operands into LDS with direct-to-LDS loads and
+# wait for them with fixed vector-memory counts (s_waitcnt vmcnt(N)).  Their
+# kernels must compile without scratch: a spilling build adds vector-memory
+# traffic and spill sequences those waits were not written for, and one such
+# build (20 bytes per lane in the five-slice pair kernel) returned wrong
+# amplitudes.  The build fails instead of shipping one.
+NO_SCRATCH_SOURCES = ("mp2_bf16.hip",)
+
+
+def scratch_sizes(remarks: str) -> dict:
+    """{kernel: scratch bytes per lane} from hipcc's kernel-resource-usage remarks."""
+    out, name = {}, None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            continue
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and name:
+            out[name] = int(m.group(1))
+    return out
+
+
 def build_hip(jobs: int = 8) -> str:
     """Compile every .hip source to an object in parallel, then link."""
     srcs = sorted(glob.glob(os.path.join(HIP_DIR, "*.hip")))
@@ -43,18 +67,31 @@ def build_hip(jobs: int = 8) -> str:
     hipcc = os.path.join(ROCM, "bin", "hipcc")
     procs = []
     objs = []
+
+    def finish(item):
+        src, proc = item
+        _, err = proc.communicate()
+        sys.stderr.write("\n".join(l for l in err.splitlines() if "remark:" not in l) + "\n")
+        if proc.returncode != 0:
+            raise RuntimeError(f"hipcc failed on {src}")
+        if os.path.basename(src) in NO_SCRATCH_SOURCES:
+            bad = {k: v for k, v in scratch_sizes(err).items() if v > 0}
+            if bad:
+                raise RuntimeError(f"{os.path.basename(src)}: kernels use scratch {bad}; the DMA "
+                                   "pipelines need spill-free kernels")
+
     for s in srcs:
         o = os.path.join(objdir, os.path.basename(s) + ".o")
         objs.append(o)
         cmd = [hipcc, *HIPCC_FLAGS, "-c", s, "-o", o]
+        if os.path.basename(s) in NO_SCRATCH_SOURCES:
+            cmd.append("-Rpass-analysis=kernel-resource-usage")
         print("+", " ".join(cmd), flush=True)
-        procs.append(subprocess.Popen(cmd))
+        procs.append((s, subprocess.Popen(cmd, stderr=subprocess.PIPE, text=True)))
         if len(procs) >= jobs:
-            if procs.pop(0).wait() != 0:
-                raise RuntimeError("hipcc failed")
-    for p in procs:
-        if p.wait() != 0:
-            raise RuntimeError("hipcc failed")
+            finish(procs.pop(0))
+    for item in procs:
+        finish(item)
     tmp = HIP_LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, HIP_LIB)

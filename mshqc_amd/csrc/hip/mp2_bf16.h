@@ -35,13 +35,19 @@ struct Bf16Operands {
     const int8_t* gel = nullptr;
     const uint16_t* gr = nullptr;
     const int8_t* ger = nullptr;
+    // FP64 self terms sum_Q Bl^Q_ia Br^Q_ia, [nself_o][nself_v] (null when
+    // not requested); see PairArgs::gself.
+    const double* gself = nullptr;
+    int nself_o = 0, nself_v = 0;
 };
 
 // Slice Bl (and Br when not null) into BF16 planes; `grad` adds the GEMM
 // layout for occupied blocks of bo_l (left) and bo_r (right) orbitals.
+// `self_terms` also forms the FP64 self terms sum_Q Bl_ia Br_ia (Br = Bl
+// when null) for i < min(ol, orr), a < min(vl, vr).
 int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int vl,
                  const double* Br, int orr, int vr, int Q, bool grad, int bo_l, int bo_r,
-                 Bf16Operands* out);
+                 bool self_terms, Bf16Operands* out);
 // Pair-tile kernel on BF16 MFMA with the shared FP64 epilogue.
 int bf16_launch_pair(mshqc_hip_ctx* ctx, int mode, const PairArgs& a, const Bf16Operands& ops);
 // Gradient contractions of one occupied block pair from the FP64 amplitude

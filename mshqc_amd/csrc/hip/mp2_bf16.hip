@@ -32,9 +32,9 @@ constexpr int KC = 32;    // contraction chunk of the gradient GEMM (one 16x16x3
 constexpr int LROW = 64;  // LDS bytes per 32-element BF16 row
 // Exponent block of the gradient-GEMM operands along k.  Its tier-0 sums
 // reach 2^24, where the MFMA accumulation can lose a last bit (2^-24 of the
-// block sum); with three slices the truncation already sits at that level,
-// so the 256-wide block (half the FP64 flushes of 128) keeps the gradient
-// well inside its bar.
+// block sum); with three slices the truncation sits near 2^-26, so the
+// 256-wide block (half the FP64 flushes of 128) keeps the gradient well
+// inside its bar.
 constexpr int GB = 256;
 constexpr int GT = 128;   // gradient-GEMM workgroup tile (rows and columns)
 constexpr int GW = 8;     // gradient-GEMM waves per workgroup (two per SIMD, 64 x 32 each)
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restric
             const int e = block_exponent(mx);
             ex[row] = e;
             const int r = r0 + row;
-            if (r < ov) Pe[((long)blockIdx.y * o + r / v) * vE + r % v] = (int8_t)e;
+            if (r < ov) Pe[((long)blockIdx.y * o + r / v) * vE + r % v] = exp_byte(e);
         }
     }
     __syncthreads();
@@ -123,6 +123,28 @@ __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restric
 #pragma unroll
         for (int p = 0; p < S; ++p) *reinterpret_cast<uint4*>(dst + p * plane) = *reinterpret_cast<uint4*>(out[p]);
     }
+}
+
+// Self terms out[i * nv + a] = sum_q Bl[q][i][a] Br[q][i][a] in FP64 (q in
+// ascending order, fused multiply-adds), i < no, a < nv.  One thread per
+// (i, a): consecutive threads read consecutive addresses of each q plane;
+// eight independent partial chains keep several loads in flight and are
+// added in a fixed order.
+__global__ __launch_bounds__(256) void self_terms_kernel(const double* __restrict__ Bl, const double* __restrict__ Br,
+                                                         int Q, long ovl, long ovr, int vl, int vr, int no, int nv,
+                                                         double* __restrict__ out) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long)no * nv) return;
+    const int i = (int)(idx / nv), a = (int)(idx % nv);
+    const double* pl = Bl + (long)i * vl + a;
+    const double* pr = Br + (long)i * vr + a;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int q = 0;
+    for (; q + 8 <= Q; q += 8)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] = fma(pl[(long)(q + u) * ovl], pr[(long)(q + u) * ovr], acc[u]);
+    for (int u = 0; q < Q; ++q, ++u) acc[u] = fma(pl[(long)q * ovl], pr[(long)q * ovr], acc[u]);
+    out[idx] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
 // src[R][ld] with column index seg*v + b (seg < nseg, b < v)  ->
@@ -165,7 +187,7 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
 #pragma unroll
         for (int p = 0; p < S; ++p) d[p * slice + 64 * u] = sv[p];
     }
-    if (lane == 0) ex[(long)rem * ldx + row] = (int8_t)e;
+    if (lane == 0) ex[(long)rem * ldx + row] = exp_byte(e);
 }
 
 // Transposed slicing of Z[(i',a)][(j',b)] (rows i'*v + a, i' < nI; ld = ldz;
@@ -227,7 +249,7 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
         for (int p = 0; p < S; ++p)
             *reinterpret_cast<uint4*>(d + p * slice + u8) = *reinterpret_cast<uint4*>(out[p]);
     }
-    if (part == 0) ex[(long)(ip * ntv + at) * ldx + c0 + cc] = (int8_t)e;
+    if (part == 0) ex[(long)(ip * ntv + at) * ldx + c0 + cc] = exp_byte(e);
 }
 
 // ------------------------------------------------ shared MFMA helpers
@@ -671,6 +693,27 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     // reused (or handed to the next workgroup).
     dma_barrier<0>();
 
+    // Self terms: on a diagonal tile (arow = bcol) of a pair i = j the
+    // elements a = b take the FP64 sums sum_Q Bl_ia Br_ia.  Lane (h, fr)
+    // holds element a = b = arow + 32t + fr of subtile (t, t) when
+    // h = (fr >> 2) & 1, at u = fr >> 3, c = fr & 3.  They are set here, in
+    // the block accumulators, so the mirrored tiles exchanged through LDS
+    // carry them too and the epilogue needs no extra registers.
+    if (p.gself != nullptr && i == j && arow == bcol && i < p.nself_o) {
+        const bool mine = h == ((fr >> 2) & 1);
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int a = arow + 32 * t + fr;
+            const double gs = p.gself[(long)i * p.nself_v + min(a, p.nself_v - 1)];
+            const bool use = mine && a < p.nself_v;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (use && u == (fr >> 3) && c == (fr & 3)) acc64[t][t][u][c] = gs;
+        }
+    }
+
     v4d accH[2][2][4];
     if (SYM && diag) {
         // Diagonal block: wave 2 holds tile (2k+1, 2k), the mirror of wave 1's
@@ -771,7 +814,7 @@ struct SlicedMat {
 // stage, no staging registers).  The
 // gradient needs ~1e-7 relative accuracy, so products go to two FP32 tiers:
 // c = 0 (integers, exact up to the last bit above 2^23) and c >= 1 (rounding
-// near 2^-31 of the block scale); both are flushed to FP64 with the block
+// near 2^-33 of the block scale); both are flushed to FP64 with the block
 // exponents once per GB-wide k block.
 template <int S>
 __global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMat X, int K,
@@ -1048,7 +1091,7 @@ struct AllowLdsGemm {
 }  // namespace
 
 int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int vl, const double* Br,
-                 int orr, int vr, int Q, bool grad, int bo_l, int bo_r, Bf16Operands* out) {
+                 int orr, int vr, int Q, bool grad, int bo_l, int bo_r, bool self_terms, Bf16Operands* out) {
     const int S = slices;
     by_slices<AllowLds>(S, ctx);
     Bf16Operands o;
@@ -1065,13 +1108,28 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     o.vEr = Br ? round_up(vr, 128) : o.vEl;
     const size_t bl = (size_t)S * ovl * o.Qp * 2, br = (size_t)S * ovr * o.Qp * 2;
     const size_t el = (size_t)o.qblocks * ol * o.vEl, er = Br ? (size_t)o.qblocks * orr * o.vEr : 0;
-    char* w = static_cast<char*>(workspace(ctx, WS_SLICE, bl + br + el + er + 1024));
+    // self terms after the exponent tables, 256-byte aligned
+    const int nso = self_terms ? (Br ? std::min(ol, orr) : ol) : 0;
+    const int nsv = self_terms ? (Br ? std::min(vl, vr) : vl) : 0;
+    const size_t gs_off = (bl + br + el + er + 255) / 256 * 256;
+    const size_t gs_bytes = sizeof(double) * (size_t)nso * nsv;
+    char* w = static_cast<char*>(workspace(ctx, WS_SLICE, gs_off + gs_bytes + 1024));
     if (!w) return MSHQC_HIP_ERR_ALLOC;
     o.sl = reinterpret_cast<uint16_t*>(w);
     o.sr = Br ? reinterpret_cast<uint16_t*>(w + bl) : nullptr;
     o.xl = reinterpret_cast<int8_t*>(w + bl + br);
     o.xr = Br ? reinterpret_cast<int8_t*>(w + bl + br + el) : nullptr;
     HIP_TRY(ctx, hipMemsetAsync(w + bl + br, 0, el + er, ctx->stream));
+    if (nso > 0 && nsv > 0) {
+        double* gs = reinterpret_cast<double*>(w + gs_off);
+        const long n = (long)nso * nsv;
+        self_terms_kernel<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(Bl, Br ? Br : Bl, Q, ovl, Br ? ovr : ovl, vl,
+                                                                   Br ? vr : vl, nso, nsv, gs);
+        LAUNCH_CHECK(ctx);
+        o.gself = gs;
+        o.nself_o = nso;
+        o.nself_v = nsv;
+    }
     int rc = by_slices<SlicePair>(S, ctx, Bl, Q, ol, vl, o.vEl, o.Qp, const_cast<uint16_t*>(o.sl),
                                   const_cast<int8_t*>(o.xl));
     if (rc) return rc;

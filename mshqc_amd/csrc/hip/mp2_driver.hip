@@ -254,7 +254,8 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
     if (pr.precision != 0) {
         int rc = bf16_prepare(ctx, pr.precision, pr.Bl, pr.ol, pr.vl,
                               sym ? nullptr : pr.Br, pr.orr, pr.vr, pr.Q,
-                              pr.grad && (pr.dBl || pr.dBr), bo_l, bo_r, &bops);
+                              pr.grad && (pr.dBl || pr.dBr), bo_l, bo_r,
+                              pr.mode == MODE_RMP2 || pr.mode == MODE_OS, &bops);
         if (rc) return rc;
     }
 
@@ -282,6 +283,11 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
         a.ntB = ntB;
         a.nwork = (long)sg.n * ntiles;
         a.e_part = part;
+        if (pr.precision != 0) {
+            a.gself = bops.gself;
+            a.nself_o = bops.nself_o;
+            a.nself_v = bops.nself_v;
+        }
         if (pr.mode == MODE_T2) {
             a.Z = pr.t2;
         } else if (pr.grad) {
@@ -393,6 +399,20 @@ int check_dims(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v, const d
     if (Q <= 0 || o <= 0 || v <= 0)
         return set_error(ctx, MSHQC_HIP_ERR_ARG, "%s: dimensions must be positive (Q=%d o=%d v=%d)",
                          who, Q, o, v);
+    if ((long)o * v > (1L << 30) || (long)Q * o * v > (1L << 40))
+        return set_error(ctx, MSHQC_HIP_ERR_ARG, "%s: dimensions too large", who);
+    return MSHQC_HIP_OK;
+}
+
+// One spin's inputs of a UMP2 call: a spin may have no occupied or no
+// virtual orbitals (the H atom has no beta electrons), and then contributes
+// nothing; pointers of empty arrays may be null.
+int check_spin(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v, const double* eo,
+               const double* ev, const char* who) {
+    if (Q <= 0 || o < 0 || v < 0)
+        return set_error(ctx, MSHQC_HIP_ERR_ARG, "%s: need Q > 0 and o, v >= 0 (Q=%d o=%d v=%d)", who, Q, o, v);
+    if ((o > 0 && v > 0 && !B) || (o > 0 && !eo) || (v > 0 && !ev))
+        return set_error(ctx, MSHQC_HIP_ERR_ARG, "%s: null input pointer", who);
     if ((long)o * v > (1L << 30) || (long)Q * o * v > (1L << 40))
         return set_error(ctx, MSHQC_HIP_ERR_ARG, "%s: dimensions too large", who);
     return MSHQC_HIP_OK;
@@ -510,8 +530,8 @@ static int ump2_common(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb, i
                        double* deob, double* devb) {
     int rc = ctx_enter(ctx);
     if (rc) return rc;
-    if ((rc = check_dims(ctx, Ba, Q, oa, va, eoa, eva, "ump2 (alpha)"))) return rc;
-    if ((rc = check_dims(ctx, Bb, Q, ob, vb, eob, evb, "ump2 (beta)"))) return rc;
+    if ((rc = check_spin(ctx, Ba, Q, oa, va, eoa, eva, "ump2 (alpha)"))) return rc;
+    if ((rc = check_spin(ctx, Bb, Q, ob, vb, eob, evb, "ump2 (beta)"))) return rc;
     if ((rc = check_precision(ctx, precision))) return rc;
     CHECK_ARG(ctx, E != nullptr, "ump2: null E");
     Timer tm(ctx);
@@ -519,12 +539,14 @@ static int ump2_common(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb, i
     if (!slot) return MSHQC_HIP_ERR_ALLOC;
     HIP_TRY(ctx, hipMemsetAsync(slot, 0, 3 * sizeof(double), ctx->stream));
     if (grad) {
-        if (dBa) HIP_TRY(ctx, hipMemsetAsync(dBa, 0, sizeof(double) * (size_t)Q * oa * va, ctx->stream));
-        if (dBb) HIP_TRY(ctx, hipMemsetAsync(dBb, 0, sizeof(double) * (size_t)Q * ob * vb, ctx->stream));
-        if (deoa) HIP_TRY(ctx, hipMemsetAsync(deoa, 0, sizeof(double) * oa, ctx->stream));
-        if (deva) HIP_TRY(ctx, hipMemsetAsync(deva, 0, sizeof(double) * va, ctx->stream));
-        if (deob) HIP_TRY(ctx, hipMemsetAsync(deob, 0, sizeof(double) * ob, ctx->stream));
-        if (devb) HIP_TRY(ctx, hipMemsetAsync(devb, 0, sizeof(double) * vb, ctx->stream));
+        // zero-length outputs of an empty spin are not touched
+        const size_t na = (size_t)Q * oa * va, nb = (size_t)Q * ob * vb;
+        if (dBa && na) HIP_TRY(ctx, hipMemsetAsync(dBa, 0, sizeof(double) * na, ctx->stream));
+        if (dBb && nb) HIP_TRY(ctx, hipMemsetAsync(dBb, 0, sizeof(double) * nb, ctx->stream));
+        if (deoa && oa) HIP_TRY(ctx, hipMemsetAsync(deoa, 0, sizeof(double) * oa, ctx->stream));
+        if (deva && va) HIP_TRY(ctx, hipMemsetAsync(deva, 0, sizeof(double) * va, ctx->stream));
+        if (deob && ob) HIP_TRY(ctx, hipMemsetAsync(deob, 0, sizeof(double) * ob, ctx->stream));
+        if (devb && vb) HIP_TRY(ctx, hipMemsetAsync(devb, 0, sizeof(double) * vb, ctx->stream));
     }
     for (int spin = 0; spin < 2; ++spin) {
         Mp2Problem pr;
@@ -541,7 +563,8 @@ static int ump2_common(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb, i
         pr.dBl = pr.dBr = spin ? dBb : dBa;
         pr.deol = pr.deor = spin ? deob : deoa;
         pr.devl = pr.devr = spin ? devb : deva;
-        if (pr.ol >= 2 && (rc = run_pairs(ctx, pr))) return rc;
+        // same-spin pairs need two occupied and one virtual orbital
+        if (pr.ol >= 2 && pr.vl >= 1 && (rc = run_pairs(ctx, pr))) return rc;
     }
     Mp2Problem pr;
     pr.mode = MODE_OS;
@@ -565,7 +588,7 @@ static int ump2_common(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb, i
     pr.devl = deva;
     pr.deor = deob;
     pr.devr = devb;
-    if ((rc = run_pairs(ctx, pr))) return rc;
+    if (oa > 0 && va > 0 && ob > 0 && vb > 0 && (rc = run_pairs(ctx, pr))) return rc;
     if ((rc = tm.stop())) return rc;
     HIP_TRY(ctx, hipMemcpy(E, slot, 3 * sizeof(double), hipMemcpyDeviceToHost));
     return MSHQC_HIP_OK;

@@ -45,6 +45,14 @@ struct PairArgs {
     double* s_part;  // [nwork]      weighted sum of m over the tile
     double* vrow;    // [nwork][64]  weighted row sums of m (left virtual index)
     double* vcol;    // [nwork][64]  weighted column sums of m (right virtual index)
+    // Self terms (split-BF16 only; null in FP64): gself[i * nself_v + a] =
+    // sum_Q Bl^Q_ia Br^Q_ia in FP64 for i < nself_o, a < nself_v, used for the
+    // elements (ia|ia) of the pairs i = j.  With Bl = Br these sums are
+    // positive definite, so their truncated slice products err in one
+    // direction (the dropped p = q products are squares) instead of
+    // averaging out as every other element's do.
+    const double* gself;
+    int nself_o, nself_v;
 };
 
 // Per-element epilogue.  Given G = (ia|jb), H = (ib|ja) and D, returns the

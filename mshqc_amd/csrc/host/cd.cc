@@ -822,8 +822,11 @@ CholeskyUMP2Result CholeskyUMP2::compute() {
     r.n_cholesky_vectors = Q;
     r.compression_ratio = chol_.compression_ratio();
     t0 = std::chrono::high_resolution_clock::now();
-    auto Ba = transform_device(chol_, uhf_.C_alpha, na_);
-    auto Bb = transform_device(chol_, uhf_.C_beta, nb_);
+    // a spin without occupied or virtual orbitals has no B (and contributes
+    // 0, as the reference's loops over its empty ranges do)
+    std::unique_ptr<device::Buffer> Ba, Bb;
+    if (na_ > 0 && va_ > 0) Ba = transform_device(chol_, uhf_.C_alpha, na_);
+    if (nb_ > 0 && vb_ > 0) Bb = transform_device(chol_, uhf_.C_beta, nb_);
     r.time_transform_s = seconds_since(t0);
     const Vec &ea = uhf_.orbital_energies_alpha, &eb = uhf_.orbital_energies_beta;
     auto deoa = to_device(Vec(ea.begin(), ea.begin() + na_));
@@ -832,7 +835,8 @@ CholeskyUMP2Result CholeskyUMP2::compute() {
     auto devb = to_device(Vec(eb.begin() + nb_, eb.end()));
     t0 = std::chrono::high_resolution_clock::now();
     double E[3] = {0, 0, 0};
-    device::check(mshqc_hip_ump2_energy(device::ctx(), Ba->get(), Bb->get(), Q, na_, va_, nb_, vb_, deoa->get(),
+    device::check(mshqc_hip_ump2_energy(device::ctx(), Ba ? Ba->get() : nullptr, Bb ? Bb->get() : nullptr, Q,
+                                        na_, va_, nb_, vb_, deoa->get(),
                                         deva->get(), deob->get(), devb->get(), config_.precision, E),
                   "mshqc_hip_ump2_energy");
     r.time_energy_s = seconds_since(t0);

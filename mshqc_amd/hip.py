@@ -165,12 +165,21 @@ class Context:
     def jk(self, L, P, screen=False):
         """J [n,n] and K (shape of P) from L [Q,n,n] and P [n,n] / [nspin,n,n]."""
         torch = self._torch
+        if L.dim() != 3 or L.shape[1] != L.shape[2]:
+            raise ValueError(f"L: need [Q, n, n], got {tuple(L.shape)}")
         Q, n, _ = L.shape
-        nspin = 1 if P.dim() == 2 else P.shape[0]
+        if P.dim() == 2:
+            nspin = 1
+            pshape = (n, n)
+        elif P.dim() == 3 and P.shape[0] in (1, 2):
+            nspin = P.shape[0]
+            pshape = (nspin, n, n)
+        else:
+            raise ValueError(f"P: need [n, n] or [nspin, n, n] with nspin 1 or 2, got {tuple(P.shape)}")
         J = torch.empty((n, n), dtype=torch.float64, device=L.device)
         K = torch.empty_like(P)
         self._follow_stream()
-        rc = lib().mshqc_hip_jk(self.h, self._ptr(L, name="L"), Q, n, self._ptr(P, name="P"),
+        rc = lib().mshqc_hip_jk(self.h, self._ptr(L, name="L"), Q, n, self._ptr(P, pshape, "P"),
                                 nspin, int(bool(screen)), self._ptr(J), self._ptr(K))
         self._check(rc, "mshqc_hip_jk")
         return J, K

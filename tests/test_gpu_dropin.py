@@ -111,6 +111,29 @@ def test_cd_uhf_then_cd_ump2(m):
     assert nb == 9
 
 
+def test_h_atom_cd_uhf_then_cd_ump2(m):
+    """H atom (1 alpha, 0 beta), cc-pVDZ: the beta spin has no occupied
+    orbitals and the alpha spin a single one, so no same-spin or
+    opposite-spin pair exists and the CD-UMP2 correlation energy is 0, as the
+    reference's loops over the empty ranges give (cholesky_ump2.cc:299-465)."""
+    mol = m.Molecule(0, 2)
+    mol.add_atom(1, 0.0, 0.0, 0.0)
+    b, eng, chol, L, S, H = setup(m, mol)
+    cfg = m.CholeskyUHFConfig()
+    cfg.print_level = 0
+    uhf = m.CholeskyUHF(mol, b, eng, 1, 0, cfg)
+    uhf.set_cholesky_vectors(chol.get_L_vectors())
+    res = uhf.compute()
+    assert abs(res.energy_total - (-0.49927840)) < 2e-8
+    ucfg = m.CholeskyUMP2Config()
+    ucfg.print_level = 0
+    for prec in (0, 4):
+        ucfg.precision = prec
+        r = m.CholeskyUMP2(res, b, eng, ucfg, chol).compute()
+        assert r.e_corr_ss_aa == 0.0 and r.e_corr_ss_bb == 0.0 and r.e_corr_os == 0.0
+        assert r.e_total == res.energy_total
+
+
 def test_ump2_of_rhf_equals_rmp2(m):
     mol = chain(m, 4)
     b, eng, chol, L, S, H = setup(m, mol, thr=1e-5)

@@ -99,6 +99,36 @@ def test_ump2_matches_oracle(ctx, Q, oa, va, ob, vb, seed):
         check_close(a, b, 1e-12, "b")
 
 
+@pytest.mark.parametrize("ob,vb", [(0, 30), (3, 0), (1, 30)])
+@pytest.mark.parametrize("precision", [0, 4])
+def test_ump2_empty_or_single_beta(ctx, ob, vb, precision):
+    """A beta spin with no occupied (ob = 0) or no virtual (vb = 0) orbitals,
+    or a single one (no same-spin pair): its blocks give 0 and the alpha
+    same-spin energy and gradients are unchanged, as in the reference's loops
+    over empty ranges (cholesky_ump2.cc:299-465)."""
+    import torch
+
+    (Ba, eoa, eva), (Bb, eob, evb) = syn.ump2_inputs(50, 4, 30, max(ob, 1), max(vb, 1), 54)
+    Bb, eob, evb = Bb[:, :ob, :vb], eob[:ob], evb[:vb]
+    E_ref, grads_ref = oracle.ump2(Ba, Bb, eoa, eva, eob, evb, grad=True) if ob and vb else (None, None)
+    dev = to_dev(Ba, Bb, eoa, eva, eob, evb)
+    E = ctx.ump2_energy(*dev, precision=precision)
+    E2, grads = ctx.ump2_grad(*dev, precision=precision)
+    Eaa = oracle.ump2(Ba, Ba[:, :0, :], eoa, eva, eoa[:0], eva)[0] if E_ref is None else E_ref[0]
+    tol = 1e-12 if precision == 0 else 1e-10
+    for got in (E, E2):
+        assert abs(got[0] - Eaa) <= tol * abs(Eaa)
+        assert got[1] == 0.0
+        if E_ref is None:
+            assert got[2] == 0.0
+        else:
+            assert abs(got[2] - E_ref[2]) <= tol * abs(E_ref[2])
+    assert tuple(grads[1].shape) == (50, ob, vb)
+    if ob == 0 or vb == 0:
+        _, g_aa = oracle.ump2(Ba, Ba[:, :0, :], eoa, eva, eoa[:0], eva, grad=True)
+        check_close(to_np(grads[0]), g_aa[0], 1e-6 if precision else 1e-12, "dE/dBa")
+
+
 def test_ump2_identical_spins_equals_rmp2(ctx):
     B, eo, ev = syn.rmp2_inputs(60, 6, 50, 61)
     dB, deo, dev = to_dev(B, eo, ev)

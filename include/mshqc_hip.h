@@ -186,6 +186,27 @@ int mshqc_hip_rmp2_grad(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v
                         double* E, double* dEdB, double* dE_deps_o, double* dE_deps_v);
 
 /*
+ * The FP64 CD-RMP2 energy of mshqc_hip_rmp2_energy (precision 0), allocating
+ * nothing.  It has no counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops).  Same kernels and summation order, so the same energy bit for bit.
+ *   - Scratch: a fixed 64 MiB device array per GPU, held until the process
+ *     ends, holding the pair and tile lists and one partial per (pair,
+ *     64 x 64 tile): about 8 (t + 2p + p t) bytes for p = o(o+1)/2 pairs and
+ *     t = T(T+1)/2 tiles, T = ceil(v/64).  Sizes beyond it are rejected with
+ *     MSHQC_HIP_ERR_ARG (o=100, v=1100 needs 7 MiB).
+ *   - E is a DEVICE pointer to one double, overwritten.
+ *   - Asynchronous: the kernels are queued on the context's stream and the
+ *     call returns without synchronising; E is ready when that stream is.
+ *   - Every call on one GPU uses that GPU's one scratch array, so calls that
+ *     overlap on one GPU (two streams, two contexts, two threads) race.  Each
+ *     call writes all of the scratch it reads, so nothing is cleared between
+ *     calls.
+ */
+int mshqc_hip_rmp2_energy_noalloc(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v,
+                                  const double* eps_o, const double* eps_v, double* E);
+
+/*
  * Amplitudes t2[i][j][a][b] = (ia|jb)/D (0 where |D| < 1e-12): the t2 tensor
  * CholeskyRMP2 stores for CholeskyRMP3 (cholesky_rmp2.cc:145-148,160).
  */

@@ -104,6 +104,9 @@ void* workspace(mshqc_hip_ctx* ctx, int slot, size_t bytes);
 
 // Enter the ctx's device; every entry point calls this first.
 int ctx_enter(mshqc_hip_ctx* ctx);
+// Looks up, once per GPU, the device address of the fixed scratch array of
+// mshqc_hip_rmp2_energy_noalloc (noalloc.hip); `device` must be current.
+int noalloc_init(int device);
 // Synchronise the ctx stream and turn asynchronous faults into a status.
 int ctx_finish(mshqc_hip_ctx* ctx);
 

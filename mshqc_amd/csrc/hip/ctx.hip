@@ -111,6 +111,14 @@ int mshqc_hip_ctx_create(int device, mshqc_hip_ctx** out) {
         return MSHQC_HIP_ERR_DEVICE;
     }
     ctx->stream = ctx->own_stream;
+    // the no-allocation entry point's scratch address on this GPU
+    if (noalloc_init(device) != MSHQC_HIP_OK) {
+        hipEventDestroy(ctx->ev0);
+        hipEventDestroy(ctx->ev1);
+        hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return MSHQC_HIP_ERR_DEVICE;
+    }
     *out = ctx;
     return MSHQC_HIP_OK;
 }

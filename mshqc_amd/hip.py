@@ -51,6 +51,7 @@ _SIGS = {
     "mshqc_hip_rmp2_grad": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D,
                             _vp, _vp, _vp],
     "mshqc_hip_rmp2_t2": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp],
+    "mshqc_hip_rmp2_energy_noalloc": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _vp],
     "mshqc_hip_ump2_energy": [_vp, _vp, _vp] + [_c_int] * 5 + [_vp] * 4 + [_c_int, _D],
     "mshqc_hip_ump2_grad": [_vp, _vp, _vp] + [_c_int] * 5 + [_vp] * 4 + [_c_int, _D]
     + [_vp] * 6,
@@ -259,6 +260,17 @@ class Context:
                                          ctypes.byref(E))
         self._check(rc, "mshqc_hip_rmp2_energy")
         return E.value
+
+    def rmp2_energy_noalloc(self, B, eps_o, eps_v, E):
+        """FP64 CD-RMP2 energy into the caller's device tensor E (one float64),
+        allocating nothing; queued on torch's current stream, not synchronised."""
+        Q, o, v = B.shape
+        self._follow_stream()
+        rc = lib().mshqc_hip_rmp2_energy_noalloc(self.h, self._ptr(B, name="B"), Q, o, v,
+                                                 self._ptr(eps_o, (o,), "eps_o"),
+                                                 self._ptr(eps_v, (v,), "eps_v"), self._ptr(E, (1,), "E"))
+        self._check(rc, "mshqc_hip_rmp2_energy_noalloc")
+        return E
 
     def rmp2_grad(self, B, eps_o, eps_v, precision=PREC_FP64, want=(True, True, True)):
         """Returns (E, dE/dB, dE/deps_o, dE/deps_v); entries not wanted are None."""

@@ -242,6 +242,23 @@ int mshqc_hip_ump2_grad(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
                         double* dE_deps_oa, double* dE_deps_va,
                         double* dE_deps_ob, double* dE_deps_vb);
 
+/*
+ * UMP2 amplitudes of every spin block, replacing
+ * CholeskyUMP2::compute_t2_amplitudes (src/mp2/cholesky_ump2.cc:471-556):
+ *   t2_aa[i][j][a][b] = g_ijab / (e_i + e_j - e_a - e_b)   (alpha, all i, j)
+ *   t2_bb likewise for beta,
+ *   t2_ab[i][j][a][b] = sum_Q Ba^Q_ia Bb^Q_jb / D  (i alpha, j beta)
+ * with g_ijab = sum_Q B^Q_ia B^Q_jb of the same spin and 0 where
+ * |D| < 1e-12; not antisymmetrised, as the reference stores them.  DEVICE
+ * outputs [oa][oa][va][va], [ob][ob][vb][vb], [oa][ob][va][vb]; any may be
+ * NULL, and an empty block is skipped.  FP64.
+ */
+int mshqc_hip_ump2_t2(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
+                      int Q, int oa, int va, int ob, int vb,
+                      const double* eps_oa, const double* eps_va,
+                      const double* eps_ob, const double* eps_vb,
+                      double* t2_aa, double* t2_bb, double* t2_ab);
+
 /* ---------------------------------------------------------------- OMP2 */
 
 /*

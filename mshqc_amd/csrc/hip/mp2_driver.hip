@@ -114,7 +114,7 @@ struct Mp2Problem {
     bool grad = false;
     double *dBl = nullptr, *dBr = nullptr;
     double *deol = nullptr, *devl = nullptr, *deor = nullptr, *devr = nullptr;
-    double* t2 = nullptr;      // MODE_T2 output
+    double* t2 = nullptr;      // MODE_T2 / MODE_T2OS output
 };
 
 inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
@@ -122,7 +122,8 @@ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // Runs the pair kernel over all (pair, tile) work of a problem, either in one
 // launch (energy, t2) or per occupied block pair with the gradient GEMMs.
 int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
-    const bool sym = pr.mode != MODE_OS;
+    const bool sym = pair_mode_sym(pr.mode);
+    const bool t2mode = pr.mode == MODE_T2 || pr.mode == MODE_T2OS;
     const int ntA = ceil_div(pr.vl, TILE), ntB = ceil_div(pr.vr, TILE);
     std::vector<int2> tiles;
     for (int A = 0; A < ntA; ++A)
@@ -156,7 +157,7 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
         Seg sg{0, 0, 0, pr.ol, 0, pr.orr};
         for (int i = 0; i < pr.ol; ++i)
             for (int j = 0; j < pr.orr; ++j) {
-                if (pr.mode == MODE_T2) {
+                if (t2mode) {
                     pairs.push_back(make_int2(i, j));
                     wts.push_back(1.0);
                 } else if (sym) {
@@ -288,7 +289,7 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
             a.nself_o = bops.nself_o;
             a.nself_v = bops.nself_v;
         }
-        if (pr.mode == MODE_T2) {
+        if (t2mode) {
             a.Z = pr.t2;
         } else if (pr.grad) {
             a.s_part = part + maxwork;
@@ -307,7 +308,7 @@ int run_pairs(mshqc_hip_ctx* ctx, const Mp2Problem& pr) {
                                    : bf16_launch_pair(ctx, pr.mode, a, bops);
         }
         if (rc) return rc;
-        if (pr.mode != MODE_T2) {
+        if (!t2mode) {
             rc = reduce_sum(ctx, part, a.nwork, pr.e_slot, true);
             if (rc) return rc;
         }
@@ -501,6 +502,45 @@ int mshqc_hip_rmp2_grad(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v
     if ((rc = tm.stop())) return rc;
     HIP_TRY(ctx, hipMemcpy(E, slot, sizeof(double), hipMemcpyDeviceToHost));
     return MSHQC_HIP_OK;
+}
+
+int mshqc_hip_ump2_t2(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb, int Q, int oa, int va, int ob,
+                      int vb, const double* eps_oa, const double* eps_va, const double* eps_ob,
+                      const double* eps_vb, double* t2_aa, double* t2_bb, double* t2_ab) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    if ((rc = check_spin(ctx, Ba, Q, oa, va, eps_oa, eps_va, "ump2_t2 (alpha)"))) return rc;
+    if ((rc = check_spin(ctx, Bb, Q, ob, vb, eps_ob, eps_vb, "ump2_t2 (beta)"))) return rc;
+    Timer tm(ctx);
+    // (Bl, Br, ol, vl, orr, vr, eps ..., out) of the three blocks
+    struct Blk {
+        const double *Bl, *Br;
+        int ol, vl, orr, vr;
+        const double *eol, *evl, *eor, *evr;
+        double* out;
+        int mode;
+    } blks[3] = {{Ba, Ba, oa, va, oa, va, eps_oa, eps_va, eps_oa, eps_va, t2_aa, MODE_T2},
+                 {Bb, Bb, ob, vb, ob, vb, eps_ob, eps_vb, eps_ob, eps_vb, t2_bb, MODE_T2},
+                 {Ba, Bb, oa, va, ob, vb, eps_oa, eps_va, eps_ob, eps_vb, t2_ab, MODE_T2OS}};
+    for (const Blk& b : blks) {
+        if (!b.out || b.ol == 0 || b.vl == 0 || b.orr == 0 || b.vr == 0) continue;
+        Mp2Problem pr;
+        pr.mode = b.mode;
+        pr.Bl = b.Bl;
+        pr.Br = b.Br;
+        pr.ol = b.ol;
+        pr.vl = b.vl;
+        pr.orr = b.orr;
+        pr.vr = b.vr;
+        pr.Q = Q;
+        pr.eol = b.eol;
+        pr.evl = b.evl;
+        pr.eor = b.eor;
+        pr.evr = b.evr;
+        pr.t2 = b.out;
+        if ((rc = run_pairs(ctx, pr))) return rc;
+    }
+    return tm.stop();
 }
 
 int mshqc_hip_rmp2_t2(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v,

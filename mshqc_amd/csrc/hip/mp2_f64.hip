@@ -11,7 +11,7 @@ constexpr int TILE = 64, KC = 16, NT = 256, LDSP = TILE + 1;
 
 template <int MODE>
 __global__ __launch_bounds__(NT) void pair_f64_kernel(PairArgs p) {
-    constexpr bool SYM = MODE != MODE_OS;
+    constexpr bool SYM = pair_mode_sym(MODE);
     constexpr int NP = SYM ? 4 : 2;
     __shared__ double S[NP][KC][LDSP];
     __shared__ double red_row[2][TILE], red_col[2][TILE], red_e[4], red_s[4];
@@ -112,6 +112,7 @@ int launch_pair_f64(mshqc_hip_ctx* ctx, int mode, const PairArgs& a) {
         case MODE_SS: pair_f64_kernel<MODE_SS><<<grid, NT, 0, ctx->stream>>>(a); break;
         case MODE_OS: pair_f64_kernel<MODE_OS><<<grid, NT, 0, ctx->stream>>>(a); break;
         case MODE_T2: pair_f64_kernel<MODE_T2><<<grid, NT, 0, ctx->stream>>>(a); break;
+        case MODE_T2OS: pair_f64_kernel<MODE_T2OS><<<grid, NT, 0, ctx->stream>>>(a); break;
         default: return set_error(ctx, MSHQC_HIP_ERR_ARG, "pair kernel: bad mode %d", mode);
     }
     LAUNCH_CHECK(ctx);

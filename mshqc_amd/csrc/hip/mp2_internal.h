@@ -47,7 +47,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& p, int i, int j, d
         for (int r = 0; r < 4; ++r) rowq[a][r] = 0.0;
     }
     double* zrow_base = nullptr;
-    if (MODE == MODE_T2) zrow_base = p.Z + ((long)i * p.orr + j) * p.vl * p.vr;
+    if (MODE == MODE_T2 || MODE == MODE_T2OS) zrow_base = p.Z + ((long)i * p.orr + j) * p.vl * p.vr;
 #pragma unroll
     for (int sa = 0; sa < 2; ++sa)
 #pragma unroll
@@ -59,7 +59,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& p, int i, int j, d
                 if (a >= p.vl || b >= p.vr) continue;
                 const double D = ei - p.evl[a] - p.evr[b];
                 const double G = accG[sa][tb][r];
-                const double H = (MODE == MODE_OS) ? 0.0 : accH[sa][tb][r];
+                const double H = pair_mode_sym(MODE) ? accH[sa][tb][r] : 0.0;
                 double e1, m1, z1;
                 pair_terms<MODE>(G, H, D, e1, m1, z1);
                 double ee = e1, q = m1, e2 = 0.0, m2 = 0.0, z2 = 0.0;
@@ -74,7 +74,7 @@ __device__ __forceinline__ void pair_epilogue(const PairArgs& p, int i, int j, d
                     rowq[sa][r] += q;
                     colq[tb] += q;
                 }
-                if (MODE == MODE_T2) {
+                if (MODE == MODE_T2 || MODE == MODE_T2OS) {
                     const bool ok = fabs(D) >= 1e-12;
                     zrow_base[(long)a * p.vr + b] = ok ? G / D : 0.0;
                     if (mirror) zrow_base[(long)b * p.vr + a] = ok ? H / D : 0.0;

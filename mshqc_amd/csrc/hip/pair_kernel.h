@@ -17,7 +17,14 @@ enum PairMode {
     MODE_SS = 1,    // E += 1/4 (G - H)^2/D (cholesky_ump2.cc:315-350)
     MODE_OS = 2,    // E += G^2/D           (cholesky_ump2.cc:434-457)
     MODE_T2 = 3,    // writes t2[i][j][a][b] = G/D (cholesky_rmp2.cc:145-148)
+    MODE_T2OS = 4,  // the same for two operands, all tiles (A, B), no mirror:
+                    // t2_ab of CholeskyUMP2::compute_t2_amplitudes
+                    // (cholesky_ump2.cc:533-552)
 };
+
+// Symmetric modes take the tiles A <= B with the mirrored H block; MODE_OS
+// and MODE_T2OS take every tile (A, B) and no mirror.
+__host__ __device__ constexpr bool pair_mode_sym(int mode) { return mode != MODE_OS && mode != MODE_T2OS; }
 
 struct PairArgs {
     const double* Bl;  // [Q][ol][vl]
@@ -81,7 +88,7 @@ __device__ __forceinline__ void pair_terms(double G, double H, double D, double&
         e = 0.25 * (G - H) * u;
         m = 0.25 * u * u;
         z = 2.0 * u;
-    } else {
+    } else {  // MODE_OS, MODE_T2OS
         const double t = G / D;
         e = G * t;
         m = t * t;

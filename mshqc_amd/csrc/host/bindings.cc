@@ -30,6 +30,14 @@ Mat np_to_mat(const arr& a) {
     return M;
 }
 
+// A flattened [d0][d1][d2][d3] amplitude block as a 4-D array (None when empty).
+py::object t2_array(const Vec& t, int d0, int d1, int d2, int d3) {
+    if (t.empty()) return py::none();
+    py::array_t<double> a({d0, d1, d2, d3});
+    std::copy(t.begin(), t.end(), a.mutable_data());
+    return std::move(a);
+}
+
 py::array_t<double> vec_to_np(const Vec& v) {
     py::array_t<double> a((py::ssize_t)v.size());
     std::copy(v.begin(), v.end(), a.mutable_data());
@@ -319,7 +327,9 @@ PYBIND11_MODULE(_mshqc, m) {
              py::arg("uhf_result"), py::arg("basis"), py::arg("integrals"), py::arg("config") = CholeskyUMP3Config())
         .def(py::init<const CholeskyUMP2&, const CholeskyUMP3Config&>(), py::arg("ump2_solver"),
              py::arg("config") = CholeskyUMP3Config())
-        .def("compute", &CholeskyUMP3::compute);
+        .def("compute", &CholeskyUMP3::compute)
+        .def("initialize_cholesky", &CholeskyUMP3::initialize_cholesky)
+        .def("transform_cholesky_vectors", &CholeskyUMP3::transform_cholesky_vectors);
 
     // ---------------------------------------------------------- OMP2
     // bindings.cc:311-331 (OMP2Result) and 495-519 of the reference
@@ -443,7 +453,18 @@ PYBIND11_MODULE(_mshqc, m) {
         .def(py::init<const SCFResult&, const BasisSet&, std::shared_ptr<IntegralEngine>,
                       const CholeskyUMP2Config&, const CholeskyERI&>())
         .def("compute", &CholeskyUMP2::compute, "Compute UMP2 energy using Cholesky vectors")
-        .def("get_cholesky", &CholeskyUMP2::get_cholesky, py::return_value_policy::reference_internal);
+        .def("compute_t2_amplitudes", &CholeskyUMP2::compute_t2_amplitudes)
+        .def("get_cholesky", &CholeskyUMP2::get_cholesky, py::return_value_policy::reference_internal)
+        // additions: the stored amplitudes (None before compute_t2_amplitudes)
+        .def_property_readonly("t2_aa", [](const CholeskyUMP2& u) -> py::object {
+            return t2_array(u.t2_aa(), u.n_occ_alpha(), u.n_occ_alpha(), u.n_virt_alpha(), u.n_virt_alpha());
+        })
+        .def_property_readonly("t2_bb", [](const CholeskyUMP2& u) -> py::object {
+            return t2_array(u.t2_bb(), u.n_occ_beta(), u.n_occ_beta(), u.n_virt_beta(), u.n_virt_beta());
+        })
+        .def_property_readonly("t2_ab", [](const CholeskyUMP2& u) -> py::object {
+            return t2_array(u.t2_ab(), u.n_occ_alpha(), u.n_occ_beta(), u.n_virt_alpha(), u.n_virt_beta());
+        });
 
     m.def("linalg_eigh", [](const arr& A) {
         Vec w;

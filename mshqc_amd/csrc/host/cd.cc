@@ -852,6 +852,38 @@ CholeskyUMP2Result CholeskyUMP2::compute() {
     return r;
 }
 
+void CholeskyUMP2::compute_t2_amplitudes() {
+    // cholesky_ump2.cc:471-556 (the reference reuses the vectors its compute()
+    // transformed; here they are transformed again, on the device)
+    if (config_.print_level > 0) std::printf("\nComputing T2 amplitudes (memory-intensive)...\n");
+    if (chol_.n_vectors() == 0) chol_.decompose_gpu(integrals_->compute_eri());
+    const int Q = chol_.n_vectors();
+    std::unique_ptr<device::Buffer> Ba, Bb;
+    if (na_ > 0 && va_ > 0) Ba = transform_device(chol_, uhf_.C_alpha, na_);
+    if (nb_ > 0 && vb_ > 0) Bb = transform_device(chol_, uhf_.C_beta, nb_);
+    const Vec &ea = uhf_.orbital_energies_alpha, &eb = uhf_.orbital_energies_beta;
+    auto deoa = to_device(Vec(ea.begin(), ea.begin() + na_));
+    auto deva = to_device(Vec(ea.begin() + na_, ea.end()));
+    auto deob = to_device(Vec(eb.begin(), eb.begin() + nb_));
+    auto devb = to_device(Vec(eb.begin() + nb_, eb.end()));
+    const size_t naa = (size_t)na_ * na_ * va_ * va_, nbb = (size_t)nb_ * nb_ * vb_ * vb_,
+                 nab = (size_t)na_ * nb_ * va_ * vb_;
+    device::Buffer taa(naa), tbb(nbb), tab(nab);
+    device::check(mshqc_hip_ump2_t2(device::ctx(), Ba ? Ba->get() : nullptr, Bb ? Bb->get() : nullptr, Q, na_, va_,
+                                    nb_, vb_, deoa->get(), deva->get(), deob->get(), devb->get(),
+                                    naa ? taa.get() : nullptr, nbb ? tbb.get() : nullptr,
+                                    nab ? tab.get() : nullptr),
+                  "mshqc_hip_ump2_t2");
+    // blocks with an empty index range stay zero, as the reference's setZero()
+    t2_aa_.assign(naa, 0.0);
+    t2_bb_.assign(nbb, 0.0);
+    t2_ab_.assign(nab, 0.0);
+    if (naa && Ba) taa.download(t2_aa_.data(), naa);
+    if (nbb && Bb) tbb.download(t2_bb_.data(), nbb);
+    if (nab && Ba && Bb) tab.download(t2_ab_.data(), nab);
+    if (config_.print_level > 0) std::printf("T2 amplitudes computed and stored.\n");
+}
+
 // ================================================================ CD-UMP3
 
 CholeskyUMP3::CholeskyUMP3(const SCFResult& uhf_result, const BasisSet& basis,
@@ -877,23 +909,30 @@ CholeskyUMP3::CholeskyUMP3(const CholeskyUMP2& ump2_solver, const CholeskyUMP3Co
     vb_ = ump2_solver.vb_;
 }
 
-CholeskyUMP3Result CholeskyUMP3::compute() {
-    // cholesky_ump3.cc:160-205
-    CholeskyUMP3Result r;
-    auto t_start = std::chrono::high_resolution_clock::now();
-    if (config_.print_level > 0)
-        std::printf("\n========================================\n  Cholesky UMP3 (Block-Contraction Optimized)\n"
-                    "========================================\n");
+// Device MO blocks of both spins (cholesky_ump3.cc:119-150): B = C_occ^T L
+// C_vir, Loo, Lvv and the orbital energies, for the vector count they were
+// made from.
+struct CholeskyUMP3::MoBlocks {
+    struct Spin {
+        std::unique_ptr<device::Buffer> B, Loo, Lvv, eo, ev;
+    } s[2];
+    int Q = 0;
+};
+
+void CholeskyUMP3::initialize_cholesky() {
+    // cholesky_ump3.cc:108-117
     if (chol_.n_vectors() == 0) chol_.decompose_gpu(integrals_->compute_eri());
+    transform_cholesky_vectors();
+}
+
+void CholeskyUMP3::transform_cholesky_vectors() {
+    // cholesky_ump3.cc:119-150
+    if (chol_.n_vectors() == 0) throw std::runtime_error("CholeskyUMP3: no Cholesky vectors to transform");
     if (chol_.n_basis() != nbf_) throw std::runtime_error("CholeskyUMP3: Cholesky vectors do not match the basis");
     const int n = nbf_, Q = chol_.n_vectors();
-    r.n_cholesky_vectors = Q;
-    r.compression_ratio = chol_.compression_ratio();
     const double* dL = chol_.device_vectors();
-    // MO blocks per spin (:112-150): B = C_occ^T L C_vir, Loo, Lvv
-    struct Blocks {
-        std::unique_ptr<device::Buffer> B, Loo, Lvv, eo, ev;
-    } blk[2];
+    auto blk = std::make_shared<MoBlocks>();
+    blk->Q = Q;
     for (int s = 0; s < 2; ++s) {
         const Mat& C = s == 0 ? uhf_.C_alpha : uhf_.C_beta;
         const Vec& e = s == 0 ? uhf_.orbital_energies_alpha : uhf_.orbital_energies_beta;
@@ -901,7 +940,7 @@ CholeskyUMP3Result CholeskyUMP3::compute() {
         if (o == 0 || v == 0) continue;
         auto dCo = to_device(left_cols(C, o));
         auto dCv = to_device(right_cols(C, v));
-        Blocks& b = blk[s];
+        MoBlocks::Spin& b = blk->s[s];
         b.B = std::make_unique<device::Buffer>((size_t)Q * o * v);
         b.Loo = std::make_unique<device::Buffer>((size_t)Q * o * o);
         b.Lvv = std::make_unique<device::Buffer>((size_t)Q * v * v);
@@ -914,6 +953,22 @@ CholeskyUMP3Result CholeskyUMP3::compute() {
         b.eo = to_device(Vec(e.begin(), e.begin() + o));
         b.ev = to_device(Vec(e.begin() + o, e.begin() + o + v));
     }
+    blocks_ = std::move(blk);
+}
+
+CholeskyUMP3Result CholeskyUMP3::compute() {
+    // cholesky_ump3.cc:160-205
+    CholeskyUMP3Result r;
+    auto t_start = std::chrono::high_resolution_clock::now();
+    if (config_.print_level > 0)
+        std::printf("\n========================================\n  Cholesky UMP3 (Block-Contraction Optimized)\n"
+                    "========================================\n");
+    if (chol_.n_vectors() == 0) chol_.decompose_gpu(integrals_->compute_eri());
+    if (!blocks_ || blocks_->Q != chol_.n_vectors()) transform_cholesky_vectors();
+    const int Q = chol_.n_vectors();
+    r.n_cholesky_vectors = Q;
+    r.compression_ratio = chol_.compression_ratio();
+    const auto& blk = blocks_->s;
     auto ptr = [](const std::unique_ptr<device::Buffer>& x) -> const double* { return x ? x->get() : nullptr; };
     auto t1 = std::chrono::high_resolution_clock::now();
     double E[3] = {0, 0, 0};

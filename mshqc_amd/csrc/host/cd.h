@@ -301,10 +301,25 @@ public:
                  std::shared_ptr<IntegralEngine> integrals, const CholeskyUMP2Config& config,
                  const integrals::CholeskyERI& cholesky_from_uhf);
     CholeskyUMP2Result compute();
+    // cholesky_ump2.cc:471-556: stores t2_aa, t2_bb, t2_ab (i, j, a, b) =
+    // g_ijab / D for every ordered pair (memory o^2 v^2 per block).  The
+    // vectors are decomposed first when there are none.
+    void compute_t2_amplitudes();
     const integrals::CholeskyERI& get_cholesky() const { return chol_; }
+    // The stored amplitudes, flattened [o][o'][v][v'] (empty before
+    // compute_t2_amplitudes); not part of the reference's Python surface,
+    // which keeps them private.
+    const Vec& t2_aa() const { return t2_aa_; }
+    const Vec& t2_bb() const { return t2_bb_; }
+    const Vec& t2_ab() const { return t2_ab_; }
+    int n_occ_alpha() const { return na_; }
+    int n_occ_beta() const { return nb_; }
+    int n_virt_alpha() const { return va_; }
+    int n_virt_beta() const { return vb_; }
 
 private:
     friend class CholeskyUMP3;
+    Vec t2_aa_, t2_bb_, t2_ab_;
     SCFResult uhf_;
     BasisSet basis_;
     std::shared_ptr<IntegralEngine> integrals_;
@@ -342,8 +357,16 @@ public:
                  const CholeskyUMP3Config& config = CholeskyUMP3Config());
     CholeskyUMP3(const CholeskyUMP2& ump2_solver, const CholeskyUMP3Config& config = CholeskyUMP3Config());
     CholeskyUMP3Result compute();
+    // cholesky_ump3.cc:108-117: decompose when there are no vectors, then
+    // transform_cholesky_vectors().
+    void initialize_cholesky();
+    // cholesky_ump3.cc:119-150: the per-spin MO blocks B = C_occ^T L C_vir,
+    // Loo and Lvv, kept on the device for compute().
+    void transform_cholesky_vectors();
 
 private:
+    struct MoBlocks;
+    std::shared_ptr<MoBlocks> blocks_;
     SCFResult uhf_;
     BasisSet basis_;
     std::shared_ptr<IntegralEngine> integrals_;

@@ -55,6 +55,7 @@ _SIGS = {
     "mshqc_hip_ump2_energy": [_vp, _vp, _vp] + [_c_int] * 5 + [_vp] * 4 + [_c_int, _D],
     "mshqc_hip_ump2_grad": [_vp, _vp, _vp] + [_c_int] * 5 + [_vp] * 4 + [_c_int, _D]
     + [_vp] * 6,
+    "mshqc_hip_ump2_t2": [_vp, _vp, _vp] + [_c_int] * 5 + [_vp] * 7,
 }
 
 #: Default cap (doubles) of the per-block amplitude workspaces of a context.
@@ -327,6 +328,21 @@ class Context:
             *[self._ptr(x) for x in d])
         self._check(rc, "mshqc_hip_ump2_grad")
         return tuple(E), (dBa, dBb, *d)
+
+    def ump2_t2(self, Ba, Bb, eoa, eva, eob, evb):
+        """(t2_aa [oa,oa,va,va], t2_bb [ob,ob,vb,vb], t2_ab [oa,ob,va,vb]) = g / D."""
+        torch = self._torch
+        Q, oa, va = Ba.shape
+        _, ob, vb = Bb.shape
+        kw = dict(dtype=torch.float64, device=Ba.device)
+        t = [torch.zeros(s, **kw) for s in ((oa, oa, va, va), (ob, ob, vb, vb), (oa, ob, va, vb))]
+        self._follow_stream()
+        rc = lib().mshqc_hip_ump2_t2(
+            self.h, self._ptr(Ba, name="Ba"), self._ptr(Bb, (Q, ob, vb), "Bb"), Q, oa, va, ob, vb,
+            self._ptr(eoa, (oa,)), self._ptr(eva, (va,)), self._ptr(eob, (ob,)), self._ptr(evb, (vb,)),
+            *[self._ptr(x) for x in t])
+        self._check(rc, "mshqc_hip_ump2_t2")
+        return tuple(t)
 
     def omp2_opdm(self, Ba, Bb, eoa, eva, eob, evb):
         """(E_aa, E_bb, E_ab), (Goo_a, Gvv_a, Goo_b, Gvv_b) of one OMP2 iteration."""

@@ -111,6 +111,48 @@ def test_cd_uhf_then_cd_ump2(m):
     assert nb == 9
 
 
+def test_ump2_t2_amplitudes_and_ump3_steps(m):
+    """CholeskyUMP2.compute_t2_amplitudes (cholesky_ump2.cc:471-556) on the Li
+    doublet: the stored t2_aa, t2_bb, t2_ab equal g/D of the oracle-transformed
+    vectors; CholeskyUMP3.initialize_cholesky and transform_cholesky_vectors
+    (cholesky_ump3.cc:108-150) before compute() give compute()'s energies."""
+    mol = m.Molecule(0, 2)
+    mol.add_atom(3, 0.0, 0.0, 0.0)
+    b, eng, chol, L, S, H = setup(m, mol, basis="6-31g")
+    cfg = m.CholeskyUHFConfig()
+    cfg.print_level = 0
+    uhf = m.CholeskyUHF(mol, b, eng, 2, 1, cfg)
+    uhf.set_cholesky_vectors(chol.get_L_vectors())
+    res = uhf.compute()
+    ucfg = m.CholeskyUMP2Config()
+    ucfg.print_level = 0
+    ump2 = m.CholeskyUMP2(res, b, eng, ucfg, chol)
+    assert ump2.t2_aa is None
+    ump2.compute()
+    ump2.compute_t2_amplitudes()
+    Ca, Cb = res.C_alpha, res.C_beta
+    ea, eb = res.orbital_energies_alpha, res.orbital_energies_beta
+    Ba = oracle.transform(L, Ca[:, :2], Ca[:, 2:])
+    Bb = oracle.transform(L, Cb[:, :1], Cb[:, 1:])
+    for t, X, Y, ex, ey, name in ((ump2.t2_aa, Ba, Ba, ea[:2], ea[:2], "aa"), (ump2.t2_bb, Bb, Bb, eb[:1], eb[:1], "bb"),
+                                  (ump2.t2_ab, Ba, Bb, ea[:2], eb[:1], "ab")):
+        vx = ea[2:] if name != "bb" else eb[1:]
+        vy = eb[1:] if name != "aa" else ea[2:]
+        g = np.einsum("qia,qjb->ijab", X, Y)
+        D = ex[:, None, None, None] + ey[None, :, None, None] - vx[None, None, :, None] - vy[None, None, None, :]
+        assert t.shape == g.shape
+        assert np.abs(t - g / D).max() <= 1e-12 * np.abs(g / D).max()
+    c3 = m.CholeskyUMP3Config()
+    c3.print_level = 0
+    e_direct = m.CholeskyUMP3(ump2, c3).compute()
+    u3 = m.CholeskyUMP3(ump2, c3)
+    u3.initialize_cholesky()
+    u3.transform_cholesky_vectors()
+    e_steps = u3.compute()
+    for k in ("e_mp3_ss_aa", "e_mp3_ss_bb", "e_mp3_os"):
+        assert getattr(e_steps, k) == getattr(e_direct, k)
+
+
 def test_h_atom_cd_uhf_then_cd_ump2(m):
     """H atom (1 alpha, 0 beta), cc-pVDZ: the beta spin has no occupied
     orbitals and the alpha spin a single one, so no same-spin or

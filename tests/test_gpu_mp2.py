@@ -129,6 +129,30 @@ def test_ump2_empty_or_single_beta(ctx, ob, vb, precision):
         check_close(to_np(grads[0]), g_aa[0], 1e-6 if precision else 1e-12, "dE/dBa")
 
 
+@pytest.mark.parametrize("Q,oa,va,ob,vb,seed", UCASES + [(30, 3, 70, 0, 72, 55)])
+def test_ump2_t2_matches_oracle(ctx, Q, oa, va, ob, vb, seed):
+    """mshqc_hip_ump2_t2 against g/D from oracle-transformed inputs (the
+    reference's compute_t2_amplitudes, cholesky_ump2.cc:471-556: every ordered
+    pair, not antisymmetrised), and the blocks' own energies: E_os = sum t_ab g,
+    and the same-spin 1/4 sum (g - g^T)(t - t^T) over all i, j."""
+    (Ba, eoa, eva), (Bb, eob, evb) = syn.ump2_inputs(Q, oa, va, max(ob, 1), vb, seed)
+    Bb, eob = Bb[:, :ob], eob[:ob]
+    taa, tbb, tab = (to_np(t) for t in ctx.ump2_t2(*to_dev(Ba, Bb, eoa, eva, eob, evb)))
+    E_ref = oracle.ump2(Ba, Bb, eoa, eva, eob, evb)
+    for t, X, Y, ex, ey, vx, vy, name in ((taa, Ba, Ba, eoa, eoa, eva, eva, "aa"), (tbb, Bb, Bb, eob, eob, evb, evb, "bb"),
+                                          (tab, Ba, Bb, eoa, eob, eva, evb, "ab")):
+        if t.size == 0:
+            continue
+        g = np.einsum("qia,qjb->ijab", X, Y)
+        D = ex[:, None, None, None] + ey[None, :, None, None] - vx[None, None, :, None] - vy[None, None, None, :]
+        check_close(t, g / D, 1e-12, f"t2_{name}")
+        if name == "ab":
+            assert abs(np.sum(t * g) - E_ref[2]) <= 1e-12 * abs(E_ref[2])
+        else:
+            Ess = 0.25 * np.sum((g - g.transpose(0, 1, 3, 2)) * (t - t.transpose(0, 1, 3, 2)))
+            assert abs(Ess - E_ref[0 if name == "aa" else 1]) <= 1e-12 * abs(Ess) + 1e-15
+
+
 def test_ump2_identical_spins_equals_rmp2(ctx):
     B, eo, ev = syn.rmp2_inputs(60, 6, 50, 61)
     dB, deo, dev = to_dev(B, eo, ev)

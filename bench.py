@@ -77,12 +77,33 @@ def measured_traffic(kernel: str, workload: str, precision: str):
     return None
 
 
-def cpu_baseline(B, eo, ev, npairs, seed=7):
-    """The oracle's E+grad per-pair work on a bounded sample of ordered pairs,
-    extrapolated linearly to all o^2 ordered pairs (per-pair work is uniform)."""
+def host_cpu():
+    """The host's logical CPU count and model name (from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"logical_cpus": os.cpu_count(), "model": model}
+
+
+def cpu_baseline(B, eo, ev, npairs, nref, seed=7):
+    """CPU legs on a bounded sample of ordered pairs (i, j), one pair per first
+    index i (a fixed random j each), extrapolated linearly to all o^2 ordered
+    pairs (the per-pair work is uniform):
+      * value: the oracle's E+grad per pair (the pair block G, amplitudes,
+        energy and the dE/dB row update) -- the same work as one GPU step;
+      * reference_energy_loop: the reference's energy-only loop as written
+        (cholesky_rmp2.cc:118-155, rank-1 updates, t2 store left out) on the
+        first `nref` of the sampled pairs."""
     import oracle
 
     Q, o, v = B.shape
+    npairs = min(npairs, o)
     rng = np.random.default_rng(seed)
     rows = rng.permutation(o)[:npairs]
     cols = rng.integers(0, o, size=npairs)
@@ -90,14 +111,25 @@ def cpu_baseline(B, eo, ev, npairs, seed=7):
     t0 = time.perf_counter()
     oracle.rmp2_grad_pairs(B, eo, ev, pairs)
     dt = time.perf_counter() - t0
-    total = dt * (o * o) / npairs
+    nref = min(nref, npairs)
+    t0 = time.perf_counter()
+    oracle.rmp2_energy_pairs_rank1(B, eo, ev, pairs[:nref])
+    dt_ref = time.perf_counter() - t0
     return {
-        "value": total,
+        "value": dt * (o * o) / npairs,
         "unit": "s",
         "cores": oracle.num_threads(),
         "kind": "port",
-        "sample": f"{npairs} of {o * o} ordered (i,j) pairs (G, amplitudes, energy and "
-                  f"dE/dB row update per pair), {dt:.1f} s measured, extrapolated x{o * o / npairs:.0f}",
+        "host": host_cpu(),
+        "sample": f"{npairs} of {o * o} ordered (i,j) pairs ({100.0 * npairs / (o * o):.1f}%; G, amplitudes, "
+                  f"energy and dE/dB row update per pair), {dt:.1f} s measured, extrapolated x{o * o / npairs:.0f}",
+        "reference_energy_loop": {
+            "value": dt_ref * (o * o) / nref,
+            "unit": "s",
+            "what": "energy only, the reference's rank-1 pair loop (cholesky_rmp2.cc:118-155) restated, "
+                    "no t2 store",
+            "sample": f"{nref} of {o * o} ordered pairs, {dt_ref:.1f} s measured, extrapolated x{o * o / nref:.0f}",
+        },
     }
 
 
@@ -181,13 +213,15 @@ def mp2_line(args, results, precs, W3, config, energy_of, grads_of):
     peak = FP64_PEAK_TFLOPS if main_p == 0 else BF16_PEAK_TFLOPS
     pair_tflops = W / (pair_ms_step * 1e-3) / 1e12 if pair_ms_step > 0 else 0.0
     hw_factor = 1 if main_p == 0 else main_p * (main_p + 1) // 2
+    gemm_hw = 1 if main_p == 0 else 6  # the gradient GEMMs keep min(k, 3) slices: 6 products
     line = base_line(args, r["sec"], prec_name(main_p), config, W3, r["prof"]["step_ms"])
     line["energy"] = energy_of(r["out"])
     line["kernels"] = {
         "pair_kernel": {"launches_per_step": npair_launch / steps, "ms_per_step": pair_ms_step,
-                        "algorithmic_flop_per_step": W},
+                        "algorithmic_flop_per_step": W, "hardware_products_per_flop": hw_factor},
         "gemm": {"launches_per_step": ngemm / steps, "ms_per_step": gemm_ms_step,
-                 "algorithmic_flop_per_step": 2 * W},
+                 "algorithmic_flop_per_step": 2 * W, "hardware_products_per_flop": gemm_hw,
+                 "note": "includes the slicing of the amplitude block"},
     }
     line["roofline"] = {
         "kernel": "pair-block (ia|jb) kernel with fused denominator epilogue",
@@ -195,9 +229,14 @@ def mp2_line(args, results, precs, W3, config, energy_of, grads_of):
         "achieved": pair_tflops,
         "peak": peak,
         "unit": "TFLOP/s",
-        "frac": (pair_tflops * hw_factor) / peak,
+        # achieved counts algorithmic flops only; in the split form each one
+        # costs hw_factor BF16 MFMA products
+        "frac": pair_tflops / peak,
+        "frac_hardware_products": pair_tflops * hw_factor / peak,
+        "achieved_vs_fp64_peak": pair_tflops / FP64_PEAK_TFLOPS,
         "traffic": measured_traffic("pair", args.workload, prec_name(main_p)),
-        "traffic_unit": "bytes per launch (HBM read + write)",
+        "traffic_unit": "bytes per launch beyond L2: FETCH_SIZE (misses of the L2, served by the MALL or "
+                        "HBM) + WRITE_SIZE, each scaled by a known-size copy",
     }
     if len(precs) > 1:
         other = results[precs[1]]
@@ -212,6 +251,7 @@ def mp2_line(args, results, precs, W3, config, energy_of, grads_of):
             "abs_dE": abs(Ea - Eb),
             "rel_dE": abs(Ea - Eb) / abs(Eb),
             "speedup": other["sec"] / r["sec"],
+            "steps": len(other["prof"]["step_ms"]),
             "grad_rel_err": num / den,
         }
     return line
@@ -263,17 +303,20 @@ def run_mp2(args, torch, hip, syn, w):
     if args.copy_calibration:
         calibrate(torch, args.copy_calibration)
     results = {}
-    for p in precs:
-        log(f"[bench] E+grad {prec_name(p)}: warmup {args.warmup}, steps {args.steps}")
-        sec, out, prof = timed(torch, ctx, step_of(p), args.steps, args.warmup)
+    for k, p in enumerate(precs):
+        # the comparison leg: at most 5 timed steps (its per-step times vary by
+        # well under 1%), so the default run stays within a few minutes
+        steps, warmup = (args.steps, args.warmup) if k == 0 else (min(args.steps, 5), min(args.warmup, 1))
+        log(f"[bench] E+grad {prec_name(p)}: warmup {warmup}, steps {steps}")
+        sec, out, prof = timed(torch, ctx, step_of(p), steps, warmup)
         log(f"[bench] {prec_name(p)}: {sec:.4f} s/step, E = {energy_of(out)!r}, pair kernel {prof[0]}, "
             f"gemm {prof[1]}")
         results[p] = dict(sec=sec, out=out, prof=prof)
     config["precision"] = prec_name(precs[0])
     line = mp2_line(args, results, precs, W3, config, energy_of, grads_of)
     if w["kind"] == "rmp2" and not args.no_cpu:
-        log(f"[bench] cpu baseline on {args.cpu_pairs} sampled pairs")
-        line["cpu_baseline"] = cpu_baseline(B, eo, ev, args.cpu_pairs)
+        log(f"[bench] cpu baseline on {args.cpu_pairs} sampled pairs ({args.cpu_ref_pairs} for the reference loop)")
+        line["cpu_baseline"] = cpu_baseline(B, eo, ev, args.cpu_pairs, args.cpu_ref_pairs)
     return line
 
 
@@ -484,7 +527,10 @@ def main():
     ap.add_argument("--precision", default="bf16x4", help="fp64 | bf16xK (K slices)")
     ap.add_argument("--compare", default="fp64", help="second precision timed beside it ('' for none)")
     ap.add_argument("--workload", default="mp2_o100")
-    ap.add_argument("--cpu-pairs", type=int, default=16)
+    ap.add_argument("--cpu-pairs", type=int, default=100,
+                    help="ordered pairs (at most one per first index) timed by the CPU E+grad leg")
+    ap.add_argument("--cpu-ref-pairs", type=int, default=100,
+                    help="of those, pairs timed by the reference's energy-only loop")
     ap.add_argument("--zblock", type=float, default=0,
                     help="cap (doubles) of the gradient amplitude block per occupied block pair "
                          "(0: the library default)")

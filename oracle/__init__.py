@@ -173,6 +173,19 @@ def cholesky(eri, thr, maxvec=None):
     return L[:nv].copy()
 
 
+def rmp2_energy_pairs_rank1(B, eps_o, eps_v, pairs):
+    """The reference's energy loop as written (cholesky_rmp2.cc:118-155:
+    rank-1 updates per Cholesky vector, then the epilogue; no t2 store) over
+    the listed ordered pairs; returns their summed energy.  For timing."""
+    B, eo, ev = _c(B), _c(eps_o), _c(eps_v)
+    Q, o, v = B.shape
+    pr = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+    fn = _lib().orc_rmp2_energy_pairs_rank1
+    fn.restype = ctypes.c_double
+    fn.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _D, _I, ctypes.c_int]
+    return fn(_p(B), Q, o, v, _p(eo), _p(ev), pr.ctypes.data_as(_I), len(pr))
+
+
 def rmp2_grad_pairs(B, eps_o, eps_v, pairs):
     """Per-pair E+grad work for listed ordered pairs (distinct first indices);
     returns (E over the pairs, partial dE/dB)."""

@@ -32,6 +32,8 @@
  *   orc_transform          pinned: numpy einsum at tiny n
  *   orc_cholesky           pinned: exact reconstruction at thr -> 0, reference
  *                          build (oracle/_ref)
+ *   orc_rmp2_energy_pairs_rank1  (timing baseline) pinned: equals
+ *                          orc_rmp2_energy summed over all ordered pairs
  */
 #include <math.h>
 #include <stdlib.h>
@@ -718,4 +720,48 @@ double orc_rmp2_grad_pairs(const double* B, int Q, int o, int v,
     for (int p = 0; p < npairs; ++p) nsum_add(&tot, pe[p]);
     free(pe);
     return nsum_val(&tot);
+}
+
+/*
+ * The reference's energy loop as it is written (cholesky_rmp2.cc:118-155),
+ * for the CPU timing baseline: for each listed ordered pair (i, j) the v x v
+ * block Int_dir (column-major, as Eigen holds it) is built by Q rank-1
+ * updates Int_dir += B^k_i B^k_j^T, column by column, then the epilogue adds
+ * (2 (ia|jb) - (ib|ja)) t_ij^ab with the |D| < 1e-12 skip.  The t2 store of
+ * the reference is left out (at o = 100, v = 1100 it is 97 GB).  OpenMP over
+ * the listed pairs with a reduction, as the reference's collapse(2) loop.
+ * Returns the sum of the listed pair energies.
+ */
+double orc_rmp2_energy_pairs_rank1(const double* B, int Q, int o, int v,
+                                   const double* eo, const double* ev,
+                                   const int* pairs, int npairs) {
+    double e_corr = 0.0;
+#pragma omp parallel reduction(+ : e_corr)
+    {
+        double* G = (double*)malloc(sizeof(double) * (size_t)v * v);
+#pragma omp for schedule(dynamic)
+        for (int p = 0; p < npairs; ++p) {
+            const int i = pairs[2 * p], j = pairs[2 * p + 1];
+            memset(G, 0, sizeof(double) * (size_t)v * v);
+            for (int k = 0; k < Q; ++k) {
+                const double* x = B + ((size_t)k * o + i) * v;
+                const double* y = B + ((size_t)k * o + j) * v;
+                for (int b = 0; b < v; ++b) {
+                    double* col = G + (size_t)b * v;
+                    const double yb = y[b];
+                    for (int a = 0; a < v; ++a) col[a] += x[a] * yb;
+                }
+            }
+            for (int a = 0; a < v; ++a)
+                for (int b = 0; b < v; ++b) {
+                    const double denom = eo[i] + eo[j] - ev[a] - ev[b];
+                    if (fabs(denom) < 1e-12) continue;
+                    const double iajb = G[a + (size_t)v * b], ibja = G[b + (size_t)v * a];
+                    const double t = iajb / denom;
+                    e_corr += (2.0 * iajb - ibja) * t;
+                }
+        }
+        free(G);
+    }
+    return e_corr;
 }

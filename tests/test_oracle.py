@@ -140,3 +140,17 @@ def test_cholesky_exact_and_threshold():
     d = np.diag(M - L2.T @ L2)
     assert d.max() < 1e-2
     assert oracle.cholesky(eri, 1e9).shape[0] == 0
+
+
+def test_rank1_reference_loop_equals_energy():
+    """The timing baseline's restatement of the reference loop (rank-1 updates,
+    cholesky_rmp2.cc:118-155) over all ordered pairs equals the oracle energy,
+    and over a subset equals the sum of those pairs' energies."""
+    B, eo, ev = syn.rmp2_inputs(40, 5, 23, 17)
+    E = oracle.rmp2_energy(B, eo, ev)
+    o = B.shape[1]
+    allp = np.array([(i, j) for i in range(o) for j in range(o)])
+    assert abs(oracle.rmp2_energy_pairs_rank1(B, eo, ev, allp) - E) <= 1e-12 * abs(E)
+    sub = allp[[1, 7, 12]]
+    want = sum(oracle.rmp2_grad_pairs(B, eo, ev, sub[k:k + 1])[0] for k in range(len(sub)))
+    assert abs(oracle.rmp2_energy_pairs_rank1(B, eo, ev, sub) - want) <= 1e-12 * abs(want)

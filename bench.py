@@ -13,6 +13,9 @@ Other workloads (`--workload`):
              occupied-factor K (C_occ 600x60) and the AO->MO transform to
              B^Q_ia; one step = those three calls
   mp2_o60, mp2_o40  smaller CD-RMP2 E+grad workloads
+  transform_o100  AO->MO transform B = C_occ^T L C_vir at n=1200, Q=4000,
+             o=100, v=1100 (L of 46 GB generated on the device), split-BF16
+             (4 slices) beside FP64, with the |dE| it adds to the energy
 
 Prints ONE JSON line (this build runs on one GPU only).
 """
@@ -365,6 +368,65 @@ def run_jk(args, torch, hip, syn, w):
     return line
 
 
+def run_transform(args, torch, hip, syn, w):
+    """The AO->MO transform at the precision workload's shape: one step = one
+    mshqc_hip_transform call; L = sym(0.1 N(0,1) exp(-|mu-nu|/50)) generated
+    on the device from a seeded torch generator (46 GB, too large to build on
+    the host in a bench run), C = Q factor of a seeded Gaussian n x n."""
+    ctx = hip.context(0)
+    Q, n, o = w["Q"], w["n"], w["o"]
+    v = n - o
+    log(f"[bench] generating {args.workload}: Q={Q} n={n} o={o} v={v} on the device")
+    g = torch.Generator(device="cuda").manual_seed(w["seed"])
+    idx = torch.arange(n, device="cuda", dtype=torch.float64)
+    env = torch.exp(-(idx[:, None] - idx[None, :]).abs() / 50.0)
+    L = torch.empty((Q, n, n), dtype=torch.float64, device="cuda")
+    for q0 in range(0, Q, 100):
+        A = 0.1 * torch.randn((min(100, Q - q0), n, n), generator=g, dtype=torch.float64, device="cuda") * env
+        L[q0:q0 + A.shape[0]] = 0.5 * (A + A.transpose(1, 2))
+    del A
+    C = np.linalg.qr(np.random.default_rng(w["seed"]).standard_normal((n, n)))[0]
+    Co = torch.from_numpy(np.ascontiguousarray(C[:, :o])).cuda()
+    Cv = torch.from_numpy(np.ascontiguousarray(C[:, o:])).cuda()
+    eo, ev = syn.gen_eps(o, v, np.random.default_rng(w["seed"] + 1))
+    deo, dev = torch.from_numpy(eo).cuda(), torch.from_numpy(ev).cuda()
+    p = parse_precision(args.precision)
+    p = p if p else 4
+    res = {}
+    for prec in (p, 0):
+        sec, B, prof = timed(torch, ctx, lambda: ctx.transform(L, Co, Cv, precision=prec), args.steps, args.warmup)
+        res[prec] = (sec, B, prof)
+        log(f"[bench] transform {prec_name(prec)}: {sec * 1e3:.2f} ms/step, GEMMs {prof[1]}")
+    E = {prec: ctx.rmp2_energy(res[prec][1], deo, dev) for prec in res}
+    flop = 2.0 * Q * n * n * o + 2.0 * Q * n * o * v
+    bytes_alg = 8.0 * Q * n * n + 8.0 * Q * o * v
+    sec = res[p][0]
+    config = {"workload": f"{args.workload}: AO->MO transform B = C_occ^T L C_vir", "Q": Q, "n": n, "o": o, "v": v,
+              "parallelism": "single GPU", "precision": prec_name(p)}
+    line = base_line(args, sec, prec_name(p), config, flop, res[p][2]["step_ms"])
+    # At k slices the GEMMs issue k(k+1)/2 BF16 products per flop: 10 at
+    # k = 4 is 444 hardware flop per algorithmic byte, past the ridge of
+    # 2.5 PF/s / 8 TB/s, so the dominant GEMM is priced against the MFMA peak;
+    # the HBM view of the whole call is kept beside it.
+    n_gemm, gemm_ms = res[p][2][1]
+    gemm_s = gemm_ms * 1e-3 / args.steps
+    achieved = flop / gemm_s / 1e12
+    hbm = bytes_alg / sec / 1e9
+    line["roofline"] = {"kernel": f"gemm_bf16_nt_kernel<{p}> (both transform GEMMs of every Q chunk)",
+                        "bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / BF16_PEAK_TFLOPS,
+                        "frac_hardware_products": achieved * p * (p + 1) // 2 / BF16_PEAK_TFLOPS,
+                        "gemm_seconds_per_step": gemm_s, "traffic": None,
+                        "hbm_view": {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBS,
+                                     "algorithmic_bytes": bytes_alg,
+                                     "note": "whole call; L read once (8 Q n^2) + B written once (8 Q o v)"}}
+    line["compare"] = {"precision": "f64", "seconds": res[0][0], "speedup": res[0][0] / sec,
+                       "tflops": flop / res[0][0] / 1e12,
+                       "B_rel_err": (torch.abs(res[p][1] - res[0][1]).max() / torch.abs(res[0][1]).max()).item(),
+                       "energy_f64_B": E[0], "energy_split_B": E[p], "abs_dE": abs(E[p] - E[0])}
+    return line
+
+
 def omp2_flops(Q, oa, va, ob, vb, swapped):
     """FP64 flops the OMP2 amplitude/OPDM call executes: the (ia|kb) GEMMs
     over all ordered occupied pairs of each spin block (twice for the
@@ -561,6 +623,8 @@ def main():
         line = run_rmp3(args, torch, hip, syn, w)
     elif w["kind"] == "ump3":
         line = run_ump3(args, torch, hip, syn, w)
+    elif w["kind"] == "transform":
+        line = run_transform(args, torch, hip, syn, w)
     else:
         line = run_jk(args, torch, hip, syn, w)
     print(json.dumps(line), flush=True)

@@ -155,10 +155,16 @@ int mshqc_hip_half_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n,
  * Replaces CholeskyRMP2::transform_vectors (src/mp2/cholesky_rmp2.cc:68-92)
  * and CholeskyUMP2::transform_cholesky_vectors (src/mp2/cholesky_ump2.cc:
  * 244-293, called once per spin).  Computed occupied-first.
+ * precision 0: FP64 MFMA.  precision k in [2, 4]: split-BF16 GEMMs with k
+ * slices of L, C_occ, C_vir and the half-transformed T^Q = L^Q C_occ, one
+ * exponent per row and 256-wide block of the contracted AO index, the
+ * first product class exact in FP32 per block, blocks summed in FP64; the
+ * relative error of B is near 2^-(9k - 1) of its block scale (k = 4: about
+ * 1e-10).  L is symmetric, so T = C_occ^T L^Q reads L^Q row-wise.
  */
 int mshqc_hip_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n,
                         const double* Cocc, int o, const double* Cvir, int v,
-                        double* B);
+                        int precision, double* B);
 
 /* ---------------------------------------------------------------- RMP2 */
 

@@ -9,6 +9,7 @@
 #include <cmath>
 
 #include "common.h"
+#include "mp2_bf16.h"
 
 namespace {
 
@@ -398,12 +399,23 @@ int mshqc_hip_half_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, 
 }
 
 int mshqc_hip_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* Cocc,
-                        int o, const double* Cvir, int v, double* B) {
+                        int o, const double* Cvir, int v, int precision, double* B) {
     int rc = ctx_enter(ctx);
     if (rc) return rc;
     CHECK_ARG(ctx, L && Cocc && Cvir && B, "transform: null pointer");
     CHECK_ARG(ctx, Q > 0 && n > 0 && o > 0 && v > 0, "transform: bad dimensions");
+    if (precision != MSHQC_PREC_FP64 && (precision < 2 || precision > BF16_TRANSFORM_SLICES))
+        return set_error(ctx, MSHQC_HIP_ERR_ARG,
+                         "transform: precision must be 0 (FP64) or a slice count in [2, %d], got %d",
+                         BF16_TRANSFORM_SLICES, precision);
     hipEventRecord(ctx->ev0, ctx->stream);
+    if (precision != MSHQC_PREC_FP64) {
+        if ((rc = bf16_transform(ctx, L, Q, n, Cocc, o, Cvir, v, precision, B))) return rc;
+        hipEventRecord(ctx->ev1, ctx->stream);
+        if ((rc = ctx_finish(ctx))) return rc;
+        hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+        return MSHQC_HIP_OK;
+    }
     const long nn = (long)n * n, no = (long)n * o;
     const int qc = q_chunk(Q, no, kTmpCap);
     double* X = static_cast<double*>(workspace(ctx, WS_TMP, sizeof(double) * (size_t)qc * no));

@@ -8,6 +8,8 @@ constexpr int BF16_MAX_SLICES = 6;
 // Slices of the gradient contractions: 3 keep the relative gradient error
 // near 1e-7, below the 1e-6 bar, at 6 instead of up to 21 BF16 products.
 constexpr int BF16_GRAD_SLICES = 3;
+// Largest slice count of the split-BF16 AO->MO transform.
+constexpr int BF16_TRANSFORM_SLICES = 4;
 
 // Device-resident BF16 slices of the B operands of one MP2 problem.
 struct Bf16Operands {
@@ -48,6 +50,10 @@ struct Bf16Operands {
 int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int vl,
                  const double* Br, int orr, int vr, int Q, bool grad, int bo_l, int bo_r,
                  bool self_terms, Bf16Operands* out);
+// AO->MO transform B^Q = C_occ^T L^Q C_vir (B: [Q][o][v]) on split-BF16
+// GEMMs with S slices per operand, occupied first, in chunks of Q.
+int bf16_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* Cocc, int o,
+                   const double* Cvir, int v, int S, double* B);
 // Pair-tile kernel on BF16 MFMA with the shared FP64 epilogue.
 int bf16_launch_pair(mshqc_hip_ctx* ctx, int mode, const PairArgs& a, const Bf16Operands& ops);
 // Gradient contractions of one occupied block pair from the FP64 amplitude

@@ -816,13 +816,28 @@ struct SlicedMat {
 // c = 0 (integers, exact up to the last bit above 2^23) and c >= 1 (rounding
 // near 2^-33 of the block scale); both are flushed to FP64 with the block
 // exponents once per GB-wide k block.
+// Output row m of C is stored at C + m * ldc, or, when mdiv > 0, at
+// C + (m / mdiv) * ld1 + (m % mdiv) * ldc (the transform writes its rows
+// (i, q) into the [q][i][a] layout of B).
+struct OutRows {
+    long ldc;
+    int mdiv;
+    long ld1;
+};
+
+// Stages of the GEMM's LDS ring: three (loads two stages ahead) while they
+// fit 160 KiB, else two (loads one stage ahead).
+constexpr int gemm_nbuf(int S) { return 3 * 2 * S * GT * LROW <= 160 * 1024 ? 3 : 2; }
+
 template <int S>
 __global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMat X, int K,
-                                                               double* __restrict__ C, long ldc) {
+                                                               double* __restrict__ C, OutRows out) {
     constexpr long PANEL = (long)GT * LROW;   // bytes of one slice panel (128 rows x 32 k)
     constexpr long STAGE = 2L * S * PANEL;    // both operands, all slices
-    constexpr int NBUF = 3;
+    constexpr int NBUF = gemm_nbuf(S);
+    constexpr int AHEAD = NBUF - 1;           // stages loaded ahead of the one in use
     constexpr int PER = 2 * S * (GT / 16) / GW; // 16-row DMA loads per wave and stage
+    constexpr int WAITN = (AHEAD - 1) * PER;  // loads left in flight at a stage barrier
     static_assert(PER == 2 * S, "each wave loads one 16-row block of every operand slice");
     extern __shared__ __align__(16) char lds[];
     int mt, ntile;
@@ -868,10 +883,10 @@ __global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, Slic
         }
     const int rr = lane & 15, fs = lane >> 4;
 #pragma unroll
-    for (int g = 0; g < S; ++g) issue_part(0, g);
+    for (int a = 0; a < AHEAD; ++a)
 #pragma unroll
-    for (int g = 0; g < S; ++g) issue_part(1, g);
-    dma_barrier<PER>();
+        for (int g = 0; g < S; ++g) issue_part(a, g);
+    dma_barrier<WAITN>();
     for (int kb = 0; kb < nkb; ++kb) {
         // Exponents of this block, fetched early and without branches (rows
         // past the matrix read the last row's: their outputs are discarded),
@@ -899,7 +914,7 @@ __global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, Slic
                 for (int t = 0; t < 2; ++t) b[q][t] = lds_frag(pb + q * PANEL, wn + 16 * t + rr, fs);
 #pragma unroll
             for (int p = 0; p < S; ++p) {
-                issue_part(ch + 2, p);
+                issue_part(ch + AHEAD, p);
                 v8s a[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) a[s] = lds_frag(pa + p * PANEL, wm + 16 * s + rr, fs);
@@ -911,8 +926,9 @@ __global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, Slic
                         for (int t = 0; t < 2; ++t)
                             acc[p + q ? 1 : 0][s][t] = mfma_bf16(a[s], b[q][t], acc[p + q ? 1 : 0][s][t]);
             }
-            // Chunk ch+1 must have landed; chunk ch+2 may stay in flight.
-            dma_barrier<PER>();
+            // Chunk ch+1 must have landed; with three stages chunk ch+2 may
+            // stay in flight.
+            dma_barrier<WAITN>();
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s)
@@ -936,10 +952,12 @@ __global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, Slic
         for (int r = 0; r < 4; ++r) {
             const int m = m0 + wm + 16 * s + 4 * fs + r;
             if (m >= A.rows) continue;
+            const long row = out.mdiv > 0 ? (long)(m / out.mdiv) * out.ld1 + (long)(m % out.mdiv) * out.ldc
+                                          : (long)m * out.ldc;
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 const int n = n0 + wn + 16 * t + rr;
-                if (n < X.rows) C[(long)m * ldc + n] += acc64[s][t][r];
+                if (n < X.rows) C[row + n] += acc64[s][t][r];
             }
         }
 }
@@ -958,6 +976,18 @@ int by_slices(int S, Args&&... args) {
     }
 }
 
+// The AO->MO transform takes 2 to BF16_TRANSFORM_SLICES slices.
+template <template <int> class F, typename... Args>
+int by_tslices(int S, Args&&... args) {
+    static_assert(BF16_TRANSFORM_SLICES == 4, "instantiate the transform slice counts");
+    switch (S) {
+        case 2: return F<2>::run(args...);
+        case 3: return F<3>::run(args...);
+        case 4: return F<4>::run(args...);
+        default: return -1;
+    }
+}
+
 // Gradient-GEMM operands carry at most BF16_GRAD_SLICES slices.
 template <template <int> class F, typename... Args>
 int by_gslices(int S, Args&&... args) {
@@ -971,8 +1001,8 @@ int by_gslices(int S, Args&&... args) {
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
-// Dynamic LDS of the gradient GEMM: three stages of both operands' slices.
-constexpr size_t gemm_lds(int S) { return (size_t)3 * 2 * S * GT * LROW; }
+// Dynamic LDS of the BF16 GEMM: its ring of stages of both operands' slices.
+constexpr size_t gemm_lds(int S) { return (size_t)gemm_nbuf(S) * 2 * S * GT * LROW; }
 
 template <int S>
 struct SlicePair {
@@ -1049,11 +1079,12 @@ struct PairLaunch {
 
 template <int S>
 struct GemmLaunch {
-    static int run(mshqc_hip_ctx* ctx, const SlicedMat& A, const SlicedMat& X, int K, double* C, long ldc) {
+    static int run(mshqc_hip_ctx* ctx, const SlicedMat& A, const SlicedMat& X, int K, double* C, long ldc,
+                   int mdiv = 0, long ld1 = 0) {
         if (K % GB != 0)
             return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 GEMM: K=%d is not a multiple of %d", K, GB);
         dim3 grid(grouped_grid(ceil_div(A.rows, GT), ceil_div(X.rows, GT)));
-        gemm_bf16_nt_kernel<S><<<grid, GW * 64, gemm_lds(S), ctx->stream>>>(A, X, K, C, ldc);
+        gemm_bf16_nt_kernel<S><<<grid, GW * 64, gemm_lds(S), ctx->stream>>>(A, X, K, C, OutRows{ldc, mdiv, ld1});
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -1179,6 +1210,76 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     return MSHQC_HIP_OK;
 }
 
+int bf16_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* Cocc, int o,
+                   const double* Cvir, int v, int S, double* B) {
+    if (S < 2 || S > BF16_TRANSFORM_SLICES)
+        return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 transform: %d slices (2 to %d)", S,
+                         BF16_TRANSFORM_SLICES);
+    by_tslices<AllowLdsGemm>(S, ctx);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    // Contraction width (an AO index) padded to whole exponent blocks; every
+    // operand is sliced with one exponent per row and GB-wide k block.
+    const long kw = round_up(n, GB), nkb = kw / GB;
+    const long ox = round_up(o, GT), vx = round_up(v, GT);
+    // Q chunk: about 1.5 GiB of L slices at a time.
+    const long per_q = (long)S * n * kw * 2;
+    const int qc = (int)std::max(1L, std::min((long)Q, (long)(1.5 * (1L << 30)) / per_q));
+    const long rl = (long)qc * n, rlx = round_up(rl, GT);      // L rows (q, mu) of a chunk
+    const long rt = (long)o * qc, rtx = round_up(rt, GT);      // T rows (i, q) of a chunk
+    // workspace: C_occ^T and C_vir^T slices, then per chunk the L slices, T
+    // in FP64 and the T slices, each with its exponent table
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t at = off; off += al(bytes); return at; };
+    const size_t o_cs = take((size_t)S * o * kw * 2), o_cx = take((size_t)nkb * ox);
+    const size_t o_vs = take((size_t)S * v * kw * 2), o_vx = take((size_t)nkb * vx);
+    const size_t o_ls = take((size_t)S * rl * kw * 2), o_lx = take((size_t)nkb * rlx);
+    const size_t o_t = take(sizeof(double) * (size_t)rl * o);
+    const size_t o_ts = take((size_t)S * rt * kw * 2), o_tx = take((size_t)nkb * rtx);
+    char* w = static_cast<char*>(workspace(ctx, WS_ZS, off + 256));
+    if (!w) return MSHQC_HIP_ERR_ALLOC;
+    auto u16 = [&](size_t at) { return reinterpret_cast<uint16_t*>(w + at); };
+    auto i8 = [&](size_t at) { return reinterpret_cast<int8_t*>(w + at); };
+    double* T = reinterpret_cast<double*>(w + o_t);
+    // C_occ^T: rows i, k = nu;  C_vir^T: rows a, k = mu (transposed slicing
+    // of the n x o and n x v coefficient blocks)
+    int rc = by_tslices<SliceCols>(S, ctx, Cocc, (long)o, 1, n, (int)kw, o, u16(o_cs), i8(o_cx), ox);
+    if (rc) return rc;
+    rc = by_tslices<SliceCols>(S, ctx, Cvir, (long)v, 1, n, (int)kw, v, u16(o_vs), i8(o_vx), vx);
+    if (rc) return rc;
+    const SlicedMat Ct{u16(o_cs), (long)o * kw, kw, i8(o_cx), ox, o};
+    const SlicedMat Vt{u16(o_vs), (long)v * kw, kw, i8(o_vx), vx, v};
+    for (int q0 = 0; q0 < Q; q0 += qc) {
+        const int nq = std::min(qc, Q - q0);
+        const long nl = (long)nq * n, nt = (long)o * nq;
+        // L rows (q, mu), k = nu
+        rc = by_tslices<SliceRows>(S, ctx, L + (long)q0 * n * n, (long)n, (int)nl, 1, n, (int)kw, u16(o_ls),
+                                   i8(o_lx), rlx);
+        if (rc) return rc;
+        // T[i][(q, mu)] = sum_nu C_occ[nu][i] L^q[mu][nu]  (= (L^q C_occ)[mu][i], L^q symmetric)
+        if ((rc = fill_zero(ctx, T, nt * n))) return rc;
+        const SlicedMat Ls{u16(o_ls), nl * kw, kw, i8(o_lx), rlx, (int)nl};
+        {
+            ProfScope ps(ctx, PROF_GEMM);
+            rc = by_tslices<GemmLaunch>(S, ctx, Ct, Ls, (int)kw, T, nl, 0, 0L);
+        }
+        if (rc) return rc;
+        // T rows (i, q) = row i*nq + q, k = mu
+        rc = by_tslices<SliceRows>(S, ctx, T, (long)n, (int)nt, 1, n, (int)kw, u16(o_ts), i8(o_tx), rtx);
+        if (rc) return rc;
+        // B[q][i][a] = sum_mu T[(i, q)][mu] C_vir[mu][a]: GEMM row m = i*nq + q
+        // is stored at (m / nq) * v + (m % nq) * o*v
+        double* Bq = B + (long)q0 * o * v;
+        if ((rc = fill_zero(ctx, Bq, (long)nq * o * v))) return rc;
+        const SlicedMat Ts{u16(o_ts), nt * kw, kw, i8(o_tx), rtx, (int)nt};
+        {
+            ProfScope ps(ctx, PROF_GEMM);
+            rc = by_tslices<GemmLaunch>(S, ctx, Ts, Vt, (int)kw, Bq, (long)o * v, nq, (long)v);
+        }
+        if (rc) return rc;
+    }
+    return MSHQC_HIP_OK;
+}
+
 int bf16_launch_pair(mshqc_hip_ctx* ctx, int mode, const PairArgs& a, const Bf16Operands& ops) {
     if (a.nwork <= 0) return MSHQC_HIP_OK;
     return by_slices<PairLaunch>(ops.slices, ctx, mode, a, ops);
@@ -1206,7 +1307,7 @@ int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& o, const double* 
         SlicedMat A{o.gr + (size_t)b * S * Q * o.kwbr, (long)Q * kw, kw, o.ger + (size_t)b * Qx * (o.kwbr / GB),
                     Qx, Q};
         SlicedMat X{zs, (long)R * kw, kw, zx, Rx, (int)R};
-        rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBl + (long)I0 * vl, (long)o.ol * vl);
+        rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBl + (long)I0 * vl, (long)o.ol * vl, 0, 0L);
         if (rc) return rc;
     }
     if (second && dBr) {
@@ -1223,7 +1324,7 @@ int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& o, const double* 
         SlicedMat A{o.gl + (size_t)b * S * Q * o.kwbl, (long)Q * kw, kw, o.gel + (size_t)b * Qx * (o.kwbl / GB),
                     Qx, Q};
         SlicedMat X{zs, (long)R * kw, kw, zx, Rx, (int)R};
-        rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBr + (long)J0 * vr, (long)o.orr * vr);
+        rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBr + (long)J0 * vr, (long)o.orr * vr, 0, 0L);
         if (rc) return rc;
     }
     return MSHQC_HIP_OK;

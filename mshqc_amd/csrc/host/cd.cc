@@ -648,15 +648,18 @@ SCFResult CholeskyROHF::compute() {
 
 namespace {
 
-// B [Q][o][v] on the device from the AO vectors and an SCF coefficient block.
-std::unique_ptr<device::Buffer> transform_device(const integrals::CholeskyERI& chol, const Mat& C, int o) {
+// B [Q][o][v] on the device from the AO vectors and an SCF coefficient block;
+// precision k >= 2 runs the transform on split-BF16 GEMMs with min(k, 4)
+// slices (the MP2 classes' split-BF16 form), 0 in FP64.
+std::unique_ptr<device::Buffer> transform_device(const integrals::CholeskyERI& chol, const Mat& C, int o,
+                                                 int precision = MSHQC_PREC_FP64) {
     const int n = C.rows, v = C.cols - o, Q = chol.n_vectors();
     if (chol.n_basis() != n) throw std::runtime_error("Cholesky vectors do not match the MO basis");
     auto dCo = to_device(left_cols(C, o));
     auto dCv = to_device(right_cols(C, v));
     auto B = std::make_unique<device::Buffer>((size_t)Q * o * v);
     device::check(mshqc_hip_transform(device::ctx(), chol.device_vectors(), Q, n, dCo->get(), o, dCv->get(),
-                                      v, B->get()),
+                                      v, precision > 4 ? 4 : precision, B->get()),
                   "mshqc_hip_transform");
     return B;
 }
@@ -690,7 +693,7 @@ foundation::CholeskyRMP2Result CholeskyRMP2::compute() {
     if (nocc_ <= 0 || nvirt_ <= 0) throw std::runtime_error("CholeskyRMP2: need occupied and virtual orbitals");
     const int o = nocc_, v = nvirt_, Q = chol_->n_vectors();
     auto t0 = std::chrono::high_resolution_clock::now();
-    auto B = transform_device(*chol_, rhf_.C_alpha, o);
+    auto B = transform_device(*chol_, rhf_.C_alpha, o, config_.precision);
     foundation::CholeskyRMP2Result r;
     r.time_transform_s = seconds_since(t0);
     const Vec& eps = rhf_.orbital_energies_alpha;
@@ -753,11 +756,11 @@ RMP3Result CholeskyRMP3::compute() {
     auto deo = to_device(Vec(eps.begin(), eps.begin() + o));
     auto dev = to_device(Vec(eps.begin() + o, eps.begin() + o + v));
     device::Buffer B((size_t)Q * o * v), Loo((size_t)Q * o * o), Lvv((size_t)Q * v * v);
-    device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCv->get(), v, B.get()),
+    device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCv->get(), v, MSHQC_PREC_FP64, B.get()),
                   "mshqc_hip_transform");
-    device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCo->get(), o, Loo.get()),
+    device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCo->get(), o, MSHQC_PREC_FP64, Loo.get()),
                   "mshqc_hip_transform");
-    device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCv->get(), v, dCv->get(), v, Lvv.get()),
+    device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCv->get(), v, dCv->get(), v, MSHQC_PREC_FP64, Lvv.get()),
                   "mshqc_hip_transform");
     // first-order amplitudes: the RMP2 result's when it kept them
     const size_t nt2 = (size_t)o * o * v * v;
@@ -825,8 +828,8 @@ CholeskyUMP2Result CholeskyUMP2::compute() {
     // a spin without occupied or virtual orbitals has no B (and contributes
     // 0, as the reference's loops over its empty ranges do)
     std::unique_ptr<device::Buffer> Ba, Bb;
-    if (na_ > 0 && va_ > 0) Ba = transform_device(chol_, uhf_.C_alpha, na_);
-    if (nb_ > 0 && vb_ > 0) Bb = transform_device(chol_, uhf_.C_beta, nb_);
+    if (na_ > 0 && va_ > 0) Ba = transform_device(chol_, uhf_.C_alpha, na_, config_.precision);
+    if (nb_ > 0 && vb_ > 0) Bb = transform_device(chol_, uhf_.C_beta, nb_, config_.precision);
     r.time_transform_s = seconds_since(t0);
     const Vec &ea = uhf_.orbital_energies_alpha, &eb = uhf_.orbital_energies_beta;
     auto deoa = to_device(Vec(ea.begin(), ea.begin() + na_));
@@ -944,11 +947,11 @@ void CholeskyUMP3::transform_cholesky_vectors() {
         b.B = std::make_unique<device::Buffer>((size_t)Q * o * v);
         b.Loo = std::make_unique<device::Buffer>((size_t)Q * o * o);
         b.Lvv = std::make_unique<device::Buffer>((size_t)Q * v * v);
-        device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCv->get(), v, b.B->get()),
+        device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCv->get(), v, MSHQC_PREC_FP64, b.B->get()),
                       "mshqc_hip_transform");
-        device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCo->get(), o, b.Loo->get()),
+        device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCo->get(), o, dCo->get(), o, MSHQC_PREC_FP64, b.Loo->get()),
                       "mshqc_hip_transform");
-        device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCv->get(), v, dCv->get(), v, b.Lvv->get()),
+        device::check(mshqc_hip_transform(device::ctx(), dL, Q, n, dCv->get(), v, dCv->get(), v, MSHQC_PREC_FP64, b.Lvv->get()),
                       "mshqc_hip_transform");
         b.eo = to_device(Vec(e.begin(), e.begin() + o));
         b.ev = to_device(Vec(e.begin() + o, e.begin() + o + v));
@@ -1066,10 +1069,10 @@ void CholeskyOMP2::add_two_particle_gradient(const double* Ba, const double* Bb,
             Loo = std::make_unique<device::Buffer>((size_t)Q * no * no);
             Lvv = std::make_unique<device::Buffer>((size_t)Q * nv * nv);
             device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, dCo->get(), no,
-                                              dCo->get(), no, Loo->get()),
+                                              dCo->get(), no, MSHQC_PREC_FP64, Loo->get()),
                           "mshqc_hip_transform");
             device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, dCv->get(), nv,
-                                              dCv->get(), nv, Lvv->get()),
+                                              dCv->get(), nv, MSHQC_PREC_FP64, Lvv->get()),
                           "mshqc_hip_transform");
         }
         device::Buffer dw((size_t)nv * no);

@@ -46,7 +46,7 @@ _SIGS = {
     "mshqc_hip_omp2_opdm": [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                             _D, _vp, _vp, _vp, _vp],
     "mshqc_hip_omp2_grad2p": [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp],
-    "mshqc_hip_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _vp],
+    "mshqc_hip_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp],
     "mshqc_hip_rmp2_energy": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D],
     "mshqc_hip_rmp2_grad": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D,
                             _vp, _vp, _vp],
@@ -238,7 +238,8 @@ class Context:
         self._check(rc, "mshqc_hip_half_transform")
         return X
 
-    def transform(self, L, Cocc, Cvir):
+    def transform(self, L, Cocc, Cvir, precision=PREC_FP64):
+        """B [Q, o, v] = C_occ^T L^Q C_vir; precision k in [2, 4] runs it on split-BF16 GEMMs."""
         torch = self._torch
         Q, n, _ = L.shape
         o, v = Cocc.shape[1], Cvir.shape[1]
@@ -246,7 +247,7 @@ class Context:
         self._follow_stream()
         rc = lib().mshqc_hip_transform(self.h, self._ptr(L, name="L"), Q, n,
                                        self._ptr(Cocc, (n, o), "Cocc"), o,
-                                       self._ptr(Cvir, (n, v), "Cvir"), v, self._ptr(B))
+                                       self._ptr(Cvir, (n, v), "Cvir"), v, int(precision), self._ptr(B))
         self._check(rc, "mshqc_hip_transform")
         return B
 

@@ -20,6 +20,8 @@ Named workloads (sizes):
 ``cd_n100``     ERI matrix M = L^T L from L [1500, 100*100] (pivoted Cholesky, thr 1e-6)
 ``mp3_o20``     L: Q=800, n=200; o=20, v=180 (CD-RMP3 from the MO blocks)
 ``ump3_o20``    L: Q=800, n=200; o_a=20, o_b=18 (CD-UMP3 from the MO blocks)
+``transform_o100``  L: Q=4000, n=1200; o=100, v=1100 (AO->MO transform;
+                L made on the device by bench.py, not here)
 ==============  ==========================================================
 """
 from __future__ import annotations
@@ -36,6 +38,8 @@ WORKLOADS = {
     "cd_n100": dict(kind="cd", Q=1500, n=100, thr=1e-6, seed=1007),
     "mp3_o20": dict(kind="rmp3", Q=800, n=200, o=20, seed=1008),
     "ump3_o20": dict(kind="ump3", Q=800, n=200, oa=20, ob=18, seed=1009),
+    # L generated on the device (bench.py run_transform): 46 GB
+    "transform_o100": dict(kind="transform", Q=4000, n=1200, o=100, seed=1010),
 }
 
 

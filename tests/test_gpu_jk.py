@@ -28,6 +28,43 @@ def test_transform_matches_oracle(ctx, Q, n, o, seed):
     check_close(X, X_ref, 1e-12, "X_ref")
 
 
+# Split-BF16 transform bounds (max |B - B_ref| / max |B_ref|): each operand
+# row block is cut at 2^-(9k-1) of its bound 2^e, which sits up to ~4x above
+# a typical element, so a product of two operands carries ~16 x 2^-(9k-1)
+# of its typical size; the transform is two such GEMMs.  k = 4 is then near
+# 1e-9, where the FP32 rounding of the product classes past the first
+# (2^-33 of a block sum) takes over.
+TRANSFORM_TOL = {2: 1e-3, 3: 2e-6, 4: 1e-8}
+
+
+@pytest.mark.parametrize("Q,n,o,seed", [(23, 37, 5, 11), (130, 300, 40, 12), (9, 257, 3, 13)])
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_transform_split_bf16(ctx, Q, n, o, seed, k):
+    """mshqc_hip_transform with k split-BF16 slices against the oracle's FP64
+    transform (cholesky_rmp2.cc:68-92), on ragged AO counts (37, 257 cross
+    the 256-wide exponent blocks) and Q chunks."""
+    L, Co, Cv = _inputs(Q, n, o, seed)
+    B_ref = oracle.transform(L, Co, Cv)
+    B = to_np(ctx.transform(*to_dev(L, Co, Cv), precision=k))
+    err = np.abs(B - B_ref).max() / np.abs(B_ref).max()
+    print(f"transform bf16x{k} Q={Q} n={n} o={o}: rel {err:.2e}", flush=True)
+    assert err <= TRANSFORM_TOL[k]
+
+
+def test_transform_split_bf16_energy(ctx):
+    """The four-slice transform's effect on the CD-RMP2 energy it feeds: the
+    FP64 energy of the split-BF16 B within 1e-12 relative of that of the FP64
+    B, and so far below the 1 uEh bar at this size (|E| ~ 1e2 Eh)."""
+    L, Co, Cv = _inputs(400, 260, 20, 14)
+    eo = np.sort(np.random.default_rng(15).uniform(-1.5, -0.3, 20))
+    ev = 0.1 + 3.0 * (np.arange(240) / 240) ** 1.5
+    dL, dCo, dCv, deo, dev = to_dev(L, Co, Cv, eo, ev)
+    E64 = ctx.rmp2_energy(ctx.transform(dL, dCo, dCv), deo, dev)
+    E4 = ctx.rmp2_energy(ctx.transform(dL, dCo, dCv, precision=4), deo, dev)
+    print(f"energy from the four-slice transform: {E4!r} vs {E64!r} ({(E4 - E64) / E64:.1e} rel)", flush=True)
+    assert abs(E4 - E64) <= 1e-12 * abs(E64)
+
+
 @pytest.mark.parametrize("Q,n,nspin,screen,seed", [(23, 37, 1, True, 4), (31, 50, 2, False, 5),
                                                    (40, 130, 1, False, 6)])
 def test_jk_general_density_matches_oracle(ctx, Q, n, nspin, screen, seed):

@@ -1,7 +1,8 @@
 // Fock-build pieces (J, K) and AO->MO transforms from Cholesky vectors.
 //
-//   J   : two streaming passes over L (HBM-bound): gamma_Q = <L^Q, P_tot> with
-//         wavefront-shuffle reductions, then J = sum_Q gamma_Q L^Q.
+//   J   : two streaming passes over the mu <= nu half of L (HBM-bound; L^Q
+//         and J are symmetric): gamma_Q = <L^Q, P_tot> with wavefront-shuffle
+//         reductions, then J = sum_Q gamma_Q L^Q, mirrored.
 //   K   : FP64 MFMA GEMMs, batched over Q (general density: T = P L^Q then
 //         K += L^Q T; factored density: X^Q = L^Q Y then K += X^Q X^Q^T).
 //   B   : occupied-first transform X^Q = L^Q C_occ, B^Q = X^Q^T C_vir.
@@ -13,62 +14,77 @@
 
 namespace {
 
-// One workgroup per Q: out[q] = sum_k L[q][k] * W[k] (W = P_tot), or the
-// squared Frobenius norm when W is null.  Fixed per-thread strides and a
-// fixed shuffle/LDS tree, so the result is reproducible.
-__global__ __launch_bounds__(256) void dot_per_q_kernel(const double* __restrict__ L, long nn,
+// Folds a square matrix onto its upper triangle in place, M[m][v] += M[v][m]
+// for m < v (the lower triangle is only read), so that for a symmetric L
+// <L, M> = sum_{m <= v} L[m][v] M[m][v] exactly, whatever M's symmetry.
+__global__ void fold_upper_kernel(double* __restrict__ M, int n) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)n * n) return;
+    const int m = (int)(idx / n), v = (int)(idx % n);
+    if (m < v) M[idx] += M[(long)v * n + m];
+}
+
+// One workgroup per Q: out[q] = sum_{m <= v} L[q][m][v] W[m][v] over the
+// upper triangle (W folded by fold_upper_kernel), or, with W null, the
+// squared Frobenius norm of the symmetric L^Q (off-diagonal terms twice).
+// Wave w takes rows m = w, w+4, ...; each row's upper part [m, n) is one
+// contiguous run read 64 lanes at a time.  Fixed orders throughout.
+__global__ __launch_bounds__(256) void dot_upper_kernel(const double* __restrict__ L, int n,
                                                         const double* __restrict__ W,
                                                         double* __restrict__ out) {
+    const long nn = (long)n * n;
     const double* Lq = L + (long)blockIdx.x * nn;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double s = 0.0;
-    const long n2 = nn / 2;
-    const double2* L2 = reinterpret_cast<const double2*>(Lq);
-    const bool aligned = (((uintptr_t)Lq) & 15) == 0 && (W == nullptr || (((uintptr_t)W) & 15) == 0);
-    if (aligned) {
-        const double2* W2 = reinterpret_cast<const double2*>(W);
-        // unrolled so several loads are in flight; the sum stays in k order
+    for (int m = wave; m < n; m += 4) {
+        const double* lr = Lq + (long)m * n;
+        double d = 0.0;
+        if (W) {
+            const double* wr = W + (long)m * n;
 #pragma unroll 4
-        for (long k = threadIdx.x; k < n2; k += 256) {
-            const double2 l = L2[k];
-            if (W) {
-                const double2 w = W2[k];
-                s += l.x * w.x + l.y * w.y;
-            } else {
-                s += l.x * l.x + l.y * l.y;
-            }
+            for (int v = m + lane; v < n; v += 64) d += lr[v] * wr[v];
+        } else {
+#pragma unroll 4
+            for (int v = m + lane; v < n; v += 64) d += (v == m ? 1.0 : 2.0) * lr[v] * lr[v];
         }
-        for (long k = 2 * n2 + threadIdx.x; k < nn; k += 256) s += W ? Lq[k] * W[k] : Lq[k] * Lq[k];
-    } else {
-        for (long k = threadIdx.x; k < nn; k += 256) s += W ? Lq[k] * W[k] : Lq[k] * Lq[k];
+        s += d;
     }
-    for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
+    for (int k = 32; k > 0; k >>= 1) s += __shfl_xor(s, k, 64);
     __shared__ double sh[4];
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    if (lane == 0) sh[wave] = s;
     __syncthreads();
     if (threadIdx.x == 0) out[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-// J[k] = sum_q gamma[q] L[q][k]; two consecutive k per thread.
-__global__ __launch_bounds__(256) void j_accumulate_kernel(const double* __restrict__ L, int Q,
-                                                           long nn, const double* __restrict__ gam,
-                                                           double* __restrict__ J) {
-    const long k = 2 * ((long)blockIdx.x * 256 + threadIdx.x);
-    if (k >= nn) return;
+// J[m][v] = J[v][m] = sum_q gamma[q] L[q][m][v] for m <= v: row m is
+// blockIdx.y, two consecutive v per thread; pairs wholly below the diagonal
+// return at once, so only the upper half of L is read.  The second element
+// of a pair at the row's end (odd n) re-reads the first, in bounds.
+__global__ __launch_bounds__(256) void j_accumulate_upper_kernel(const double* __restrict__ L, int Q, int n,
+                                                                 const double* __restrict__ gam,
+                                                                 double* __restrict__ J) {
+    const int m = blockIdx.y;
+    const int v0 = 2 * (blockIdx.x * 256 + threadIdx.x);
+    if (v0 >= n || v0 + 1 < m) return;
+    const long nn = (long)n * n;
+    const bool has1 = v0 + 1 < n;
+    const int off1 = has1 ? 1 : 0;
     double s0 = 0.0, s1 = 0.0;
-    if (k + 1 < nn) {
-        // unrolled so several q's loads are in flight; the sums stay in q order
+    const double* p = L + (long)m * n + v0;
+    // unrolled so several q's loads are in flight; the sums stay in q order
 #pragma unroll 8
-        for (int q = 0; q < Q; ++q) {
-            const double g = gam[q];
-            const double* p = L + (long)q * nn + k;
-            s0 += g * p[0];
-            s1 += g * p[1];
-        }
-        J[k] = s0;
-        J[k + 1] = s1;
-    } else {
-        for (int q = 0; q < Q; ++q) s0 += gam[q] * L[(long)q * nn + k];
-        J[k] = s0;
+    for (int q = 0; q < Q; ++q) {
+        const double g = gam[q];
+        s0 += g * p[(long)q * nn];
+        s1 += g * p[(long)q * nn + off1];
+    }
+    if (v0 >= m) {
+        J[(long)m * n + v0] = s0;
+        J[(long)v0 * n + m] = s0;
+    }
+    if (has1) {
+        J[(long)m * n + v0 + 1] = s1;
+        J[(long)(v0 + 1) * n + m] = s1;
     }
 }
 
@@ -91,13 +107,17 @@ __global__ void mask_zero_kernel(double* __restrict__ T, const double* __restric
     if (!(norm2[q0 + q] > thr2)) T[idx] = 0.0;
 }
 
-int j_build(mshqc_hip_ctx* ctx, const double* L, int Q, long nn, const double* Pt, double* J,
-            double* gam) {
+// J = sum_Q <L^Q, Pt> L^Q from the upper halves of the symmetric L^Q; Pt is
+// a temporary of the caller and is folded in place.
+int j_build(mshqc_hip_ctx* ctx, const double* L, int Q, int n, double* Pt, double* J, double* gam) {
     ProfScope ps(ctx, PROF_J);
-    dot_per_q_kernel<<<Q, 256, 0, ctx->stream>>>(L, nn, Pt, gam);
+    const long nn = (long)n * n;
+    fold_upper_kernel<<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(Pt, n);
     LAUNCH_CHECK(ctx);
-    const long nthr = (nn + 1) / 2;
-    j_accumulate_kernel<<<(unsigned)((nthr + 255) / 256), 256, 0, ctx->stream>>>(L, Q, nn, gam, J);
+    dot_upper_kernel<<<Q, 256, 0, ctx->stream>>>(L, n, Pt, gam);
+    LAUNCH_CHECK(ctx);
+    dim3 grid((unsigned)((n + 511) / 512), (unsigned)n);
+    j_accumulate_upper_kernel<<<grid, 256, 0, ctx->stream>>>(L, Q, n, gam, J);
     LAUNCH_CHECK(ctx);
     return MSHQC_HIP_OK;
 }
@@ -139,9 +159,9 @@ int mshqc_hip_jk(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double
     double* norm2 = gam + Q;
     sum_spins_kernel<<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(P, nspin, nn, Pt);
     LAUNCH_CHECK(ctx);
-    if ((rc = j_build(ctx, L, Q, nn, Pt, J, gam))) return rc;
+    if ((rc = j_build(ctx, L, Q, n, Pt, J, gam))) return rc;
     if (screen) {
-        dot_per_q_kernel<<<Q, 256, 0, ctx->stream>>>(L, nn, nullptr, norm2);
+        dot_upper_kernel<<<Q, 256, 0, ctx->stream>>>(L, n, nullptr, norm2);
         LAUNCH_CHECK(ctx);
     }
     for (int s = 0; s < nspin; ++s) {
@@ -223,7 +243,7 @@ int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const
     gp.M = gp.N = n;
     gp.K = m;
     if ((rc = gemm_f64(ctx, gp))) return rc;
-    if ((rc = j_build(ctx, L, Q, nn, P, J, gam))) return rc;
+    if ((rc = j_build(ctx, L, Q, n, P, J, gam))) return rc;
     for (int q0 = 0; q0 < Q; q0 += qc) {
         const int nq = std::min(qc, Q - q0);
         // X_q = L^q Y
@@ -313,7 +333,7 @@ int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const d
             if ((rc = gemm_f64(ctx, gp))) return rc;
             have_p = true;
         }
-        if ((rc = j_build(ctx, L, Q, nn, P, J, gam))) return rc;
+        if ((rc = j_build(ctx, L, Q, n, P, J, gam))) return rc;
         if (na == 0 && (rc = fill_zero(ctx, Ka, nn))) return rc;
         if (nb == 0 && (rc = fill_zero(ctx, Kb, nn))) return rc;
         for (int q0 = 0; q0 < Q; q0 += qc) {

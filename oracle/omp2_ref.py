@@ -19,6 +19,14 @@ INFRASTRUCTURE ONLY (tiny sizes).
   orbital gradient), and orthonormal rotated orbitals (tests/test_omp2.py).
   With ``two_particle=True`` (not in the reference) each spin's w also gets
   ``two_particle_grad``.
+* ``rotation_step``  the damped, level-shifted kappa of one iteration
+  (cholesky_omp2.cc:375-425).  Pinned by: -sum kappa w is the first-order
+  change of the UHF energy along C exp(-lambda kappa), i.e. w = 2 F_vo is
+  minus the energy's gradient in this convention, and the step lowers it
+  (tests/test_omp2.py, with a brute-force 4-index energy).
+* ``opdm`` off-diagonal blocks are pinned by rotating two orbitals of equal
+  energy: the rotated diagonal c^2 G_ii + s^2 G_jj + 2 c s G_ij must equal
+  the pinned dE/deps of the rotated inputs (tests/test_omp2.py).
 * ``two_particle_grad``  the two-particle term the reference's w = 2 F_vo
   leaves out (cholesky_omp2.cc:331-369 build F from the OPDM only):
   w2(c, k) = dE_MP2/dkappa_ck at fixed denominators, from dE/dB and the
@@ -90,6 +98,25 @@ def two_particle_grad(dB, Loo, Lvv):
     return np.einsum("qca,qka->ck", Lvv, dB) - np.einsum("qic,qik->ck", dB, Loo)
 
 
+def rotation_step(w, F, o):
+    """The reference's orbital step (cholesky_omp2.cc:375-425): the n x n
+    antisymmetric kappa with kappa[o + a, i] = 0.5 w[a, i] / d and
+    kappa[i, o + a] = -kappa[o + a, i], d = F[o+a, o+a] - F[i, i] + 0.2 with
+    |d| raised to at least 1e-2 (sign kept); orbitals then go to C exp(-kappa).
+    w is [v, o], F the MO-basis Fock matrix."""
+    n = F.shape[0]
+    kappa = np.zeros((n, n))
+    for a in range(n - o):
+        for i in range(o):
+            diff = F[o + a, o + a] - F[i, i] + 0.2
+            if abs(diff) < 1e-2:
+                diff = 1e-2 if diff >= 0 else -1e-2
+            val = (w[a, i] / diff) * 0.5
+            kappa[o + a, i] = val
+            kappa[i, o + a] = -val
+    return kappa
+
+
 def cd_omp2(H, L, C_a, C_b, P_a, P_b, eps_a, eps_b, na, nb, e_scf, max_iterations=50,
             energy_threshold=1e-6, gradient_threshold=1e-5, two_particle=False):
     """The CD-OMP2 loop of cholesky_omp2.cc:427-513 with numpy and the C
@@ -113,16 +140,7 @@ def cd_omp2(H, L, C_a, C_b, P_a, P_b, eps_a, eps_b, na, nb, e_scf, max_iteration
 
     def rotate(w, F, o, C):
         # :375-425: damped, level-shifted step, C <- C exp(-kappa)
-        kappa = np.zeros((n, n))
-        for a in range(n - o):
-            for i in range(o):
-                diff = F[o + a, o + a] - F[i, i] + 0.2
-                if abs(diff) < 1e-2:
-                    diff = 1e-2 if diff >= 0 else -1e-2
-                val = (w[a, i] / diff) * 0.5
-                kappa[o + a, i] = val
-                kappa[i, o + a] = -val
-        C = C @ sl.expm(-kappa)
+        C = C @ sl.expm(-rotation_step(w, F, o))
         return C, C[:, :o] @ C[:, :o].T
 
     e_prev = e_scf

@@ -172,3 +172,107 @@ def test_host_expm(m, scale):
     assert np.abs(E.T @ E - np.eye(17)).max() < 1e-13  # exp of antisymmetric is orthogonal
     G = scale * rng.standard_normal((9, 9))
     assert np.abs(m.linalg_expm(G) - sl.expm(G)).max() < 1e-12 * max(1.0, np.abs(sl.expm(G)).max())
+
+
+def _rot2(B, axis, p, q, th):
+    """Rotate rows (axis 1) or columns (axis 2) p, q of B[Q, o, v] by th:
+    new p = c p + s q, new q = -s p + c q."""
+    c, s = np.cos(th), np.sin(th)
+    B = B.copy()
+    if axis == 1:
+        x, y = B[:, p, :].copy(), B[:, q, :].copy()
+        B[:, p, :], B[:, q, :] = c * x + s * y, -s * x + c * y
+    else:
+        x, y = B[:, :, p].copy(), B[:, :, q].copy()
+        B[:, :, p], B[:, :, q] = c * x + s * y, -s * x + c * y
+    return B
+
+
+@pytest.mark.parametrize("th", [0.6, np.pi / 4, -1.1])
+def test_opdm_off_diagonal_by_rotation(th):
+    """The OPDM's off-diagonal blocks (cholesky_omp2.cc:232-326), pinned
+    without restating them: give two occupied (or two virtual) orbitals the
+    same energy and rotate them by th.  Denominators are unchanged, the
+    amplitudes rotate covariantly, so G' = R G R^T and the rotated diagonal
+    c^2 G_pp + s^2 G_qq + 2 c s G_pq must equal dE/deps of the rotated
+    orbital, which the pinned UMP2 oracle gives (its dE/deps are pinned by
+    central differences and the reference build).  A wrong sign or index in
+    G_pq fails the 2 c s G_pq term."""
+    from mshqc_amd import synthetic as syn
+
+    (Ba, eoa, eva), (Bb, eob, evb) = syn.ump2_inputs(30, 4, 9, 3, 8, 95)
+    eoa[2] = eoa[1]
+    eva[5] = eva[3]
+    eob[1] = eob[0]
+    evb[6] = evb[2]
+    G = omp2_ref.opdm(*omp2_ref.t2_blocks(Ba, Bb, eoa, eva, eob, evb))
+    c, s = np.cos(th), np.sin(th)
+    cases = [  # (spin block rotated, axis, p, q, index of G, index of dE/deps)
+        ("a", 1, 1, 2, 0, 2), ("a", 2, 3, 5, 1, 3), ("b", 1, 0, 1, 2, 4), ("b", 2, 2, 6, 3, 5)]
+    for spin, axis, p, q, gi, di in cases:
+        Ba2 = _rot2(Ba, axis, p, q, th) if spin == "a" else Ba
+        Bb2 = _rot2(Bb, axis, p, q, th) if spin == "b" else Bb
+        _, grads = oracle.ump2(Ba2, Bb2, eoa, eva, eob, evb, grad=True)
+        Gm = G[gi]
+        want = c * c * Gm[p, p] + s * s * Gm[q, q] + 2 * c * s * Gm[p, q]
+        assert abs(grads[di][p] - want) <= 1e-12 * np.abs(Gm).max(), (spin, axis)
+        assert abs(Gm[p, q]) > 1e-6 * np.abs(Gm).max()  # the off-diagonal term is not negligible
+
+
+def _uhf_energy(H, eri, Ca, Cb, na, nb):
+    """E = sum_s tr(P_s H) + 1/2 sum_s tr(P_s (J[P_a + P_b] - K[P_s])) from a
+    4-index ERI (the UHF electronic energy, without the oracle's J/K)."""
+    Pa, Pb = Ca[:, :na] @ Ca[:, :na].T, Cb[:, :nb] @ Cb[:, :nb].T
+    J = np.einsum("mnls,ls->mn", eri, Pa + Pb)
+    E = 0.0
+    for P in (Pa, Pb):
+        K = np.einsum("mlns,ls->mn", eri, P)
+        E += np.sum(P * H) + 0.5 * np.sum(P * (J - K))
+    return E
+
+
+def test_rotation_step_is_a_descent_step_with_slope_minus_kappa_w():
+    """The loop's step (cholesky_omp2.cc:375-425), pinned by the mathematics
+    of its convention: for orbitals C -> C exp(-lambda kappa) the UHF energy
+    changes at the rate -sum_ai kappa_ai w_ai, w = 2 F_vo, so w is exactly
+    minus the energy gradient in this convention (a sign or factor error in w,
+    in kappa's antisymmetry or in exp(-kappa) breaks the equality), and the
+    damped, level-shifted step lowers the energy.  The energy comes from a
+    brute-force 4-index ERI, not from the oracle's J/K."""
+    rng = np.random.default_rng(96)
+    n, na, nb, Q = 8, 3, 2, 30
+    A = rng.standard_normal((n, n))
+    H = 0.5 * (A + A.T) - 2.0 * np.eye(n)
+    L = np.empty((Q, n, n))
+    for q in range(Q):
+        X = 0.05 * rng.standard_normal((n, n))
+        L[q] = 0.5 * (X + X.T)
+    eri = np.einsum("qmn,qls->mnls", L, L)
+    # H's aufbau orbitals (the weak two-electron part keeps its level order),
+    # then a small random rotation so w is not zero
+    _, U = np.linalg.eigh(H)
+    Ca = U @ sl.expm(0.05 * (lambda X: X - X.T)(rng.standard_normal((n, n))))
+    Cb = U @ sl.expm(0.05 * (lambda X: X - X.T)(rng.standard_normal((n, n))))
+    Pa, Pb = Ca[:, :na] @ Ca[:, :na].T, Cb[:, :nb] @ Cb[:, :nb].T
+    J = np.einsum("mnls,ls->mn", eri, Pa + Pb)
+    kap, slope = [], 0.0
+    for C, P, o in ((Ca, Pa, na), (Cb, Pb, nb)):
+        F = C.T @ (H + J - np.einsum("mlns,ls->mn", eri, P)) @ C
+        w = 2.0 * F[o:, :o]
+        k = omp2_ref.rotation_step(w, F, o)
+        assert np.allclose(k, -k.T, atol=0)
+        # the descent claim below needs every level-shifted difference positive
+        assert np.all(np.diag(F)[o:, None] - np.diag(F)[None, :o] + 0.2 > 0)
+        kap.append(k)
+        slope -= np.sum(k[o:, :o] * w)
+    def cdiff(lam):
+        Ep = _uhf_energy(H, eri, Ca @ sl.expm(-lam * kap[0]), Cb @ sl.expm(-lam * kap[1]), na, nb)
+        Em = _uhf_energy(H, eri, Ca @ sl.expm(lam * kap[0]), Cb @ sl.expm(lam * kap[1]), na, nb)
+        return (Ep - Em) / (2 * lam)
+
+    fd = (4 * cdiff(5e-5) - cdiff(1e-4)) / 3  # Richardson: O(lambda^4)
+    assert abs(fd - slope) <= 1e-9 * abs(slope)
+    assert slope < 0
+    E0 = _uhf_energy(H, eri, Ca, Cb, na, nb)
+    E1 = _uhf_energy(H, eri, Ca @ sl.expm(-kap[0]), Cb @ sl.expm(-kap[1]), na, nb)
+    assert E1 < E0

@@ -181,7 +181,12 @@ PYBIND11_MODULE(_mshqc, m) {
         .def("compute_overlap", [](const IntegralEngine& e) { return mat_to_np(e.compute_overlap()); })
         .def("compute_kinetic", [](const IntegralEngine& e) { return mat_to_np(e.compute_kinetic()); })
         .def("compute_nuclear", [](const IntegralEngine& e) { return mat_to_np(e.compute_nuclear()); })
-        .def("compute_eri", &IntegralEngine::compute_eri);
+        .def("compute_eri", &IntegralEngine::compute_eri)
+        .def("compute_eri_diagonal", [](const IntegralEngine& e) { return vec_to_np(e.compute_eri_diagonal()); },
+             "Compute diagonal ERI elements (pq|pq)")
+        .def("compute_eri_column",
+             [](const IntegralEngine& e, int pivot_index) { return vec_to_np(e.compute_eri_column(pivot_index)); },
+             "Compute specific ERI column (pq|rs) for a pivot index", py::arg("pivot_index"));
 
     // ---------------------------------------------------- CholeskyERI
     py::class_<CholeskyDecompositionResult>(m, "CholeskyDecompositionResult")

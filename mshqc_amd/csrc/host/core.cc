@@ -741,6 +741,31 @@ Mat IntegralEngine::compute_overlap() const { return one_body(basis_, offset_, n
 Mat IntegralEngine::compute_kinetic() const { return one_body(basis_, offset_, nbf_, mol_, KINETIC); }
 Mat IntegralEngine::compute_nuclear() const { return one_body(basis_, offset_, nbf_, mol_, NUCLEAR); }
 
+Vec IntegralEngine::compute_eri_diagonal() const {
+    const ERITensor eri = compute_eri();
+    Vec d;
+    d.reserve((size_t)nbf_ * (nbf_ + 1) / 2);
+    for (int i = 0; i < nbf_; ++i)
+        for (int j = 0; j <= i; ++j) d.push_back(eri(i, j, i, j));
+    return d;
+}
+
+Vec IntegralEngine::compute_eri_column(int pivot_index) const {
+    int r = 0, s = 0;
+    for (int i = 0, idx = 0; i < nbf_; ++i)
+        for (int j = 0; j <= i; ++j, ++idx)
+            if (idx == pivot_index) {
+                r = i;
+                s = j;
+            }
+    const ERITensor eri = compute_eri();
+    Vec col;
+    col.reserve((size_t)nbf_ * (nbf_ + 1) / 2);
+    for (int p = 0; p < nbf_; ++p)
+        for (int q = 0; q <= p; ++q) col.push_back(eri(p, q, r, s));
+    return col;
+}
+
 ERITensor IntegralEngine::compute_eri() const {
     ERITensor eri(nbf_);
     const auto& sh = basis_.shells();

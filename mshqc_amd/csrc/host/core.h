@@ -176,6 +176,12 @@ public:
     Mat compute_kinetic() const;
     Mat compute_nuclear() const;
     ERITensor compute_eri() const;
+    // integrals.cc:976-1032 of the reference: the diagonal (pq|pq) and one
+    // column (pq|rs) over the pairs p >= q in row order (index p(p+1)/2 + q),
+    // both from the full ERI.  An out-of-range pivot index leaves (r, s) at
+    // (0, 0), as the reference's decoding loop does.
+    Vec compute_eri_diagonal() const;
+    Vec compute_eri_column(int pivot_index) const;
     int n_basis() const { return nbf_; }
 
 private:

@@ -241,3 +241,26 @@ def test_rotation_invariance(m, tmp_path):
                                                    eng.compute_nuclear(), eri)])
     for w0, w1 in zip(*out):
         assert np.abs(w0 - w1).max() < 1e-11 * np.abs(w0).max()
+
+
+def test_eri_diagonal_and_column(m):
+    """compute_eri_diagonal / compute_eri_column (integrals.cc:976-1032): the
+    pairs p >= q in row order, index p(p+1)/2 + q; the column of pivot index
+    k is (pq|rs) for the k-th pair (r, s), and the diagonal is each column's
+    own entry."""
+    mol = mol_of(m, [(1, 0, 0, 0), (1, 0, 0, 1.4)])
+    b = m.BasisSet("cc-pvdz", mol, BD)
+    eng = m.IntegralEngine(mol, b)
+    eri = eng.compute_eri().numpy()
+    n = eri.shape[0]
+    pairs = [(p, q) for p in range(n) for q in range(p + 1)]
+    d = eng.compute_eri_diagonal()
+    assert d.shape == (n * (n + 1) // 2,)
+    assert np.array_equal(d, np.array([eri[p, q, p, q] for p, q in pairs]))
+    for k in (0, 7, len(pairs) - 1):
+        r, s = pairs[k]
+        col = eng.compute_eri_column(k)
+        assert np.array_equal(col, np.array([eri[p, q, r, s] for p, q in pairs]))
+        assert col[k] == d[k]
+    # an index past the last pair leaves (r, s) = (0, 0), as the reference's loop does
+    assert np.array_equal(eng.compute_eri_column(len(pairs)), eng.compute_eri_column(0))

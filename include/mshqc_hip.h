@@ -306,6 +306,36 @@ int mshqc_hip_omp2_opdm(mshqc_hip_ctx* ctx, const double* Ba, const double* Bb,
 int mshqc_hip_omp2_grad2p(mshqc_hip_ctx* ctx, const double* G, const double* Loo,
                           const double* Lvv, int Q, int o, int v, double* w2);
 
+/*
+ * One CD-OMP2 iteration's orbital work on the device, replacing
+ * build_gfock_from_density, the gradient and the rotation of
+ * CholeskyOMP2::compute (src/mp2/cholesky_omp2.cc:331-369, 375-425,
+ * 451-490).  Per spin the orbitals are DEVICE blocks C_occ [n][o] and
+ * C_vir [n][v] (columns = orbitals; NULL for an empty block), the density
+ * P [n][n] and the MO-basis generalised Fock F [n][n].
+ *
+ * mshqc_hip_omp2_fock: with G_oo, G_vv from mshqc_hip_omp2_opdm,
+ *   P_tot,s = P_s + C_occ G_oo C_occ^T + C_vir G_vv C_vir^T,
+ *   F_s = C^T (H + J[P_tot,a + P_tot,b] - K[P_tot,s]) C,  C = [C_occ | C_vir].
+ * mshqc_hip_omp2_grad_norm: *g2 (HOST) = sum_ai (2 F[o+a][i] + w2[a][i])^2,
+ *   w2 [v][o] the optional two-particle term (NULL for none).
+ * mshqc_hip_omp2_rotate: kappa_ai = 0.5 w_ai / d, w = 2 F_vo (+ w2),
+ *   d = F_aa - F_ii + 0.2 raised to |d| >= 1e-2 (sign kept); C <- C exp(-kappa)
+ *   (in place, the exponential by scaling and squaring of a Taylor series);
+ *   P <- C_occ C_occ^T.  o = 0 or o = n leaves everything unchanged.
+ */
+int mshqc_hip_omp2_fock(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* H,
+                        const double* Cocc_a, const double* Cvir_a, int na,
+                        const double* Cocc_b, const double* Cvir_b, int nb,
+                        const double* P_a, const double* P_b,
+                        const double* Goo_a, const double* Gvv_a,
+                        const double* Goo_b, const double* Gvv_b,
+                        double* F_a, double* F_b);
+int mshqc_hip_omp2_grad_norm(mshqc_hip_ctx* ctx, int n, int o, const double* F, const double* w2,
+                             double* g2);
+int mshqc_hip_omp2_rotate(mshqc_hip_ctx* ctx, int n, int o, const double* F, const double* w2,
+                          double* Cocc, double* Cvir, double* P);
+
 /* ---------------------------------------------------------------- RMP3 */
 
 /*

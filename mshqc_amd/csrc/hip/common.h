@@ -144,6 +144,11 @@ struct GemmF64 {
 };
 int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g);
 
+// J = sum_Q tr(L^Q P_tot) L^Q and K[s] = sum_Q L^Q P_s L^Q queued on the ctx
+// stream (the work of mshqc_hip_jk without its entry checks and timing).
+int jk_general(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* P, int nspin, int screen,
+               double* J, double* K);
+
 // Square n x n matrix (row stride ld): the elements of every tile
 // (64 * tpb rows and columns per block) whose column block exceeds its row
 // block are overwritten by their transposes, so a lower-block-triangle

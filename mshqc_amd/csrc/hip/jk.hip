@@ -148,6 +148,18 @@ int mshqc_hip_jk(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double
     CHECK_ARG(ctx, L && P && J && K, "jk: null pointer");
     CHECK_ARG(ctx, Q > 0 && n > 0 && nspin >= 1 && nspin <= 2, "jk: bad dimensions");
     hipEventRecord(ctx->ev0, ctx->stream);
+    if ((rc = jk_general(ctx, L, Q, n, P, nspin, screen, J, K))) return rc;
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if ((rc = ctx_finish(ctx))) return rc;
+    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+    return MSHQC_HIP_OK;
+}
+
+}  // extern "C"
+
+int jk_general(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* P, int nspin, int screen,
+               double* J, double* K) {
+    int rc;
     const long nn = (long)n * n;
     const int qc = q_chunk(Q, nn, kTmpCap);
     double* tmp = static_cast<double*>(
@@ -210,11 +222,10 @@ int mshqc_hip_jk(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double
         }
         if ((rc = mirror_upper(ctx, Ks, n, n, 1))) return rc;
     }
-    hipEventRecord(ctx->ev1, ctx->stream);
-    if ((rc = ctx_finish(ctx))) return rc;
-    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
     return MSHQC_HIP_OK;
 }
+
+extern "C" {
 
 int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* Y, int m,
                         double* J, double* K) {

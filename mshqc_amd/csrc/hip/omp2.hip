@@ -496,3 +496,274 @@ extern "C" int mshqc_hip_omp2_grad2p(mshqc_hip_ctx* ctx, const double* G, const 
     hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
     return MSHQC_HIP_OK;
 }
+
+// ------------------------------------------------- the orbital step, on the device
+//
+// Per spin the orbitals are held as C_occ [n][o] and C_vir [n][v] (columns =
+// orbitals), the density P [n][n], the MO-basis generalised Fock F [n][n].
+namespace {
+
+__global__ void omp2_fock_ao_kernel(const double* __restrict__ H, const double* __restrict__ J,
+                                    const double* __restrict__ K, long nn, double* __restrict__ F) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nn) F[k] = (H[k] + J[k]) - K[k];
+}
+
+// A = -kappa of the reference's step (cholesky_omp2.cc:375-425) from
+// w = 2 F[o+a][i] (+ w2[a][i]) and F's diagonal; A is zero elsewhere (the
+// caller clears it).  One thread per (a, i).
+__global__ void omp2_minus_kappa_kernel(int n, int o, const double* __restrict__ F, const double* __restrict__ w2,
+                                        double* __restrict__ A) {
+    const int v = n - o;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)v * o) return;
+    const int a = (int)(idx / o), i = (int)(idx % o);
+    double w = 2.0 * F[(long)(o + a) * n + i];
+    if (w2) w += w2[idx];
+    double diff = F[(long)(o + a) * n + o + a] - F[(long)i * n + i] + 0.2;
+    if (fabs(diff) < 1e-2) diff = diff >= 0 ? 1e-2 : -1e-2;
+    const double val = (w / diff) * 0.5;
+    A[(long)(o + a) * n + i] = -val;
+    A[(long)i * n + o + a] = val;
+}
+
+// out[0] = max_j sum_i |A[i][j]| (one workgroup, fixed order).
+__global__ __launch_bounds__(256) void omp2_norm1_kernel(int n, const double* __restrict__ A, double* out) {
+    double m = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        double c = 0.0;
+        for (int i = 0; i < n; ++i) c += fabs(A[(long)i * n + j]);
+        m = fmax(m, c);
+    }
+    for (int k = 32; k > 0; k >>= 1) m = fmax(m, __shfl_xor(m, k, 64));
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+// X = s X + t I (n x n).
+__global__ void omp2_scale_shift_kernel(int n, double s, double t, double* __restrict__ X) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)n * n) return;
+    const int i = (int)(k / n), j = (int)(k % n);
+    X[k] = s * X[k] + (i == j ? t : 0.0);
+}
+
+// out[0] = sum_{a,i} (2 F[o+a][i] + w2[a][i])^2 (one workgroup, fixed order).
+__global__ __launch_bounds__(256) void omp2_grad_norm_kernel(int n, int o, const double* __restrict__ F,
+                                                             const double* __restrict__ w2, double* out) {
+    const long nvo = (long)(n - o) * o;
+    double s = 0.0;
+    for (long idx = threadIdx.x; idx < nvo; idx += 256) {
+        const int a = (int)(idx / o), i = (int)(idx % o);
+        double w = 2.0 * F[(long)(o + a) * n + i];
+        if (w2) w += w2[idx];
+        s += w * w;
+    }
+    for (int k = 32; k > 0; k >>= 1) s += __shfl_xor(s, k, 64);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+inline unsigned blocks_of(long n) { return (unsigned)((n + 255) / 256); }
+
+// C[M][N] (+)= op(A) op(X) with row strides; skipped when a dimension is 0.
+int small_gemm(mshqc_hip_ctx* ctx, const double* A, long lda, bool tA, const double* X, long ldx, bool tX,
+               double* C, long ldc, int M, int N, int K, bool acc) {
+    if (M <= 0 || N <= 0) return MSHQC_HIP_OK;
+    if (K <= 0) {  // an empty contraction: C = 0 (or unchanged when accumulating)
+        if (acc) return MSHQC_HIP_OK;
+        for (int m = 0; m < M; ++m) {
+            int rc = fill_zero(ctx, C + (long)m * ldc, N);
+            if (rc) return rc;
+        }
+        return MSHQC_HIP_OK;
+    }
+    GemmF64 g;
+    g.A = A;
+    g.lda = lda;
+    g.transA = tA;
+    g.X = X;
+    g.ldx = ldx;
+    g.transX = tX;
+    g.C = C;
+    g.ldc = ldc;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.accumulate = acc;
+    return gemm_f64(ctx, g);
+}
+
+// E = expm(A) for an n x n A (overwritten: A is scaled in place), with the
+// host expm's algorithm (core.cc): scale by 2^-sq to 1-norm <= 1/2, Taylor
+// to the first degree m whose next term is below 2^-60, Horner, square sq
+// times.  tmp holds 2 n^2 doubles; norm1 one double.
+int device_expm(mshqc_hip_ctx* ctx, int n, double* A, double* E, double* tmp, double* norm1) {
+    const long nn = (long)n * n;
+    omp2_norm1_kernel<<<1, 256, 0, ctx->stream>>>(n, A, norm1);
+    LAUNCH_CHECK(ctx);
+    double nrm = 0.0;
+    HIP_TRY(ctx, hipMemcpyAsync(&nrm, norm1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    int sq = 0;
+    while (nrm > 0.5) {
+        nrm *= 0.5;
+        ++sq;
+    }
+    int m = 1;
+    double term = nrm;
+    while (m < 30 && term * nrm / (m + 1) >= std::ldexp(1.0, -60)) {
+        term *= nrm / (m + 1);
+        ++m;
+    }
+    omp2_scale_shift_kernel<<<blocks_of(nn), 256, 0, ctx->stream>>>(n, std::ldexp(1.0, -sq), 0.0, A);
+    LAUNCH_CHECK(ctx);
+    // Horner: E = I + X/1 (I + X/2 (I + ... (I + X/m)))
+    double* cur = tmp;
+    double* nxt = tmp + nn;
+    int rc = fill_zero(ctx, cur, nn);
+    if (rc) return rc;
+    omp2_scale_shift_kernel<<<blocks_of(nn), 256, 0, ctx->stream>>>(n, 0.0, 1.0, cur);
+    LAUNCH_CHECK(ctx);
+    for (int k = m; k >= 1; --k) {
+        if ((rc = small_gemm(ctx, A, n, false, cur, n, false, nxt, n, n, n, n, false))) return rc;
+        omp2_scale_shift_kernel<<<blocks_of(nn), 256, 0, ctx->stream>>>(n, 1.0 / k, 1.0, nxt);
+        LAUNCH_CHECK(ctx);
+        std::swap(cur, nxt);
+    }
+    for (int k = 0; k < sq; ++k) {
+        if ((rc = small_gemm(ctx, cur, n, false, cur, n, false, nxt, n, n, n, n, false))) return rc;
+        std::swap(cur, nxt);
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(E, cur, sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream));
+    return MSHQC_HIP_OK;
+}
+
+// HIP events around a call's kernels (the ctx's last_ms), then the stream
+// synchronisation that turns asynchronous faults into a status.
+struct StepTimer {
+    mshqc_hip_ctx* ctx;
+    explicit StepTimer(mshqc_hip_ctx* c) : ctx(c) { hipEventRecord(ctx->ev0, ctx->stream); }
+    int stop() {
+        hipEventRecord(ctx->ev1, ctx->stream);
+        int rc = ctx_finish(ctx);
+        if (rc) return rc;
+        hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+        hipGetLastError();
+        return MSHQC_HIP_OK;
+    }
+};
+
+int omp2_step_checks(mshqc_hip_ctx* ctx, int n, int o, const double* Co, const double* Cv, const char* who) {
+    if (n <= 0 || o < 0 || o > n)
+        return set_error(ctx, MSHQC_HIP_ERR_ARG, "%s: need n > 0 and 0 <= o <= n (n=%d o=%d)", who, n, o);
+    if ((o > 0 && !Co) || (o < n && !Cv)) return set_error(ctx, MSHQC_HIP_ERR_ARG, "%s: null orbital block", who);
+    return MSHQC_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int mshqc_hip_omp2_fock(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* H,
+                                   const double* Co_a, const double* Cv_a, int na, const double* Co_b,
+                                   const double* Cv_b, int nb, const double* P_a, const double* P_b,
+                                   const double* Goo_a, const double* Gvv_a, const double* Goo_b,
+                                   const double* Gvv_b, double* F_a, double* F_b) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, L && H && P_a && P_b && F_a && F_b && Q > 0, "omp2_fock: null pointer or Q <= 0");
+    if ((rc = omp2_step_checks(ctx, n, na, Co_a, Cv_a, "omp2_fock (alpha)"))) return rc;
+    if ((rc = omp2_step_checks(ctx, n, nb, Co_b, Cv_b, "omp2_fock (beta)"))) return rc;
+    const int va = n - na, vb = n - nb;
+    CHECK_ARG(ctx, (na == 0 || Goo_a) && (va == 0 || Gvv_a) && (nb == 0 || Goo_b) && (vb == 0 || Gvv_b),
+              "omp2_fock: null density block");
+    StepTimer tm(ctx);
+    const long nn = (long)n * n;
+    // Pt [2][n][n] | J | K [2] | F_ao | T (n x n)
+    double* w = static_cast<double*>(workspace(ctx, WS_OMP2_V, sizeof(double) * (size_t)7 * nn));
+    if (!w) return MSHQC_HIP_ERR_ALLOC;
+    double *Pt = w, *J = w + 2 * nn, *K = w + 3 * nn, *Fao = w + 5 * nn, *T = w + 6 * nn;
+    const double* Co[2] = {Co_a, Co_b};
+    const double* Cv[2] = {Cv_a, Cv_b};
+    const double* P[2] = {P_a, P_b};
+    const double* Goo[2] = {Goo_a, Goo_b};
+    const double* Gvv[2] = {Gvv_a, Gvv_b};
+    double* F[2] = {F_a, F_b};
+    const int no[2] = {na, nb};
+    // P_tot = P + C_occ G_oo C_occ^T + C_vir G_vv C_vir^T (cholesky_omp2.cc:451-459)
+    for (int s = 0; s < 2; ++s) {
+        const int o = no[s], v = n - o;
+        double* Ps = Pt + s * nn;
+        HIP_TRY(ctx, hipMemcpyAsync(Ps, P[s], sizeof(double) * nn, hipMemcpyDeviceToDevice, ctx->stream));
+        if ((rc = small_gemm(ctx, Co[s], o, false, Goo[s], o, false, T, o, n, o, o, false))) return rc;
+        if ((rc = small_gemm(ctx, T, o, false, Co[s], o, true, Ps, n, n, n, o, true))) return rc;
+        if ((rc = small_gemm(ctx, Cv[s], v, false, Gvv[s], v, false, T, v, n, v, v, false))) return rc;
+        if ((rc = small_gemm(ctx, T, v, false, Cv[s], v, true, Ps, n, n, n, v, true))) return rc;
+    }
+    // H + J[Pt_a + Pt_b] - K[Pt_s] (cholesky_omp2.cc:331-369; one J for both spins)
+    if ((rc = jk_general(ctx, L, Q, n, Pt, 2, 0, J, K))) return rc;
+    for (int s = 0; s < 2; ++s) {
+        const int o = no[s], v = n - o;
+        omp2_fock_ao_kernel<<<blocks_of(nn), 256, 0, ctx->stream>>>(H, J, K + s * nn, nn, Fao);
+        LAUNCH_CHECK(ctx);
+        // F_mo = C^T F_ao C by blocks: [F_oo F_ov; F_vo F_vv]
+        double* Xo = T;                    // F_ao C_occ [n][o]
+        double* Xv = T + (long)n * o;      // F_ao C_vir [n][v]
+        if ((rc = small_gemm(ctx, Fao, n, false, Co[s], o, false, Xo, o, n, o, n, false))) return rc;
+        if ((rc = small_gemm(ctx, Fao, n, false, Cv[s], v, false, Xv, v, n, v, n, false))) return rc;
+        double* Fs = F[s];
+        if ((rc = small_gemm(ctx, Co[s], o, true, Xo, o, false, Fs, n, o, o, n, false))) return rc;
+        if ((rc = small_gemm(ctx, Co[s], o, true, Xv, v, false, Fs + o, n, o, v, n, false))) return rc;
+        if ((rc = small_gemm(ctx, Cv[s], v, true, Xo, o, false, Fs + (long)o * n, n, v, o, n, false))) return rc;
+        if ((rc = small_gemm(ctx, Cv[s], v, true, Xv, v, false, Fs + (long)o * n + o, n, v, v, n, false))) return rc;
+    }
+    return tm.stop();
+}
+
+extern "C" int mshqc_hip_omp2_grad_norm(mshqc_hip_ctx* ctx, int n, int o, const double* F, const double* w2,
+                                        double* g2) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, F && g2 && n > 0 && o >= 0 && o <= n, "omp2_grad_norm: bad arguments");
+    double* slot = static_cast<double*>(workspace(ctx, WS_MISC, sizeof(double) * 8));
+    if (!slot) return MSHQC_HIP_ERR_ALLOC;
+    omp2_grad_norm_kernel<<<1, 256, 0, ctx->stream>>>(n, o, F, w2, slot);
+    LAUNCH_CHECK(ctx);
+    HIP_TRY(ctx, hipMemcpyAsync(g2, slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return ctx_finish(ctx);
+}
+
+extern "C" int mshqc_hip_omp2_rotate(mshqc_hip_ctx* ctx, int n, int o, const double* F, const double* w2,
+                                     double* Co, double* Cv, double* P) {
+    int rc = ctx_enter(ctx);
+    if (rc) return rc;
+    CHECK_ARG(ctx, F && P, "omp2_rotate: null pointer");
+    if ((rc = omp2_step_checks(ctx, n, o, Co, Cv, "omp2_rotate"))) return rc;
+    const int v = n - o;
+    if (o == 0 || v == 0) return MSHQC_HIP_OK;  // no occupied-virtual rotation exists
+    StepTimer tm(ctx);
+    const long nn = (long)n * n;
+    // A | U | 2 n^2 expm temporaries | new C_occ, C_vir | norm
+    double* w = static_cast<double*>(workspace(ctx, WS_OMP2_T, sizeof(double) * (size_t)(5 * nn + 8)));
+    if (!w) return MSHQC_HIP_ERR_ALLOC;
+    double *A = w, *U = w + nn, *tmp = w + 2 * nn, *Cn = w + 4 * nn, *nrm = w + 5 * nn;
+    if ((rc = fill_zero(ctx, A, nn))) return rc;
+    omp2_minus_kappa_kernel<<<blocks_of((long)v * o), 256, 0, ctx->stream>>>(n, o, F, w2, A);
+    LAUNCH_CHECK(ctx);
+    if ((rc = device_expm(ctx, n, A, U, tmp, nrm))) return rc;
+    // [C_occ | C_vir] <- [C_occ | C_vir] U, by blocks of U
+    double* Con = Cn;
+    double* Cvn = Cn + (long)n * o;
+    if ((rc = small_gemm(ctx, Co, o, false, U, n, false, Con, o, n, o, o, false))) return rc;
+    if ((rc = small_gemm(ctx, Cv, v, false, U + (long)o * n, n, false, Con, o, n, o, v, true))) return rc;
+    if ((rc = small_gemm(ctx, Co, o, false, U + o, n, false, Cvn, v, n, v, o, false))) return rc;
+    if ((rc = small_gemm(ctx, Cv, v, false, U + (long)o * n + o, n, false, Cvn, v, n, v, v, true))) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(Co, Con, sizeof(double) * (size_t)n * o, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(Cv, Cvn, sizeof(double) * (size_t)n * v, hipMemcpyDeviceToDevice, ctx->stream));
+    // P = C_occ C_occ^T
+    if ((rc = small_gemm(ctx, Co, o, false, Co, o, true, P, n, n, n, o, false))) return rc;
+    return tm.stop();
+}

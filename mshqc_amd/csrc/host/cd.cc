@@ -1026,29 +1026,13 @@ CholeskyOMP2::CholeskyOMP2(const Molecule& mol, const BasisSet& basis, std::shar
         throw std::runtime_error("CholeskyOMP2 initialized with empty reused vectors");
 }
 
-void CholeskyOMP2::rotate(const Mat& w, const Mat& F_mo, int no, int nv, Mat& C, Mat& P) const {
-    // cholesky_omp2.cc:375-425: kappa_ai = 0.5 w_ai / (F_aa - F_ii + 0.2),
-    // denominators clamped to |.| >= 1e-2, C <- C exp(-kappa)
-    const double damping = 0.5, shift = 0.2;
-    Mat kappa(nbf_, nbf_);
-    for (int a = 0; a < nv; ++a)
-        for (int i = 0; i < no; ++i) {
-            double diff = F_mo(no + a, no + a) - F_mo(i, i) + shift;
-            if (std::fabs(diff) < 1e-2) diff = diff >= 0 ? 1e-2 : -1e-2;
-            const double val = (w(a, i) / diff) * damping;
-            kappa(no + a, i) = val;
-            kappa(i, no + a) = -val;
-        }
-    C = matmul(C, expm(scale(kappa, -1.0)));
-    Mat Co = left_cols(C, no);
-    P = matmul_nt(Co, Co);
-}
-
-void CholeskyOMP2::add_two_particle_gradient(const double* Ba, const double* Bb, const double* eoa,
-                                             const double* eva, const double* eob, const double* evb,
-                                             bool same_orbitals, Mat (&w)[2]) const {
+void CholeskyOMP2::two_particle_terms(const double* Ba, const double* Bb, const double* eoa, const double* eva,
+                                      const double* eob, const double* evb, bool same_orbitals,
+                                      double* const (&Co)[2], double* const (&Cv)[2],
+                                      std::unique_ptr<device::Buffer> (&w2)[2]) const {
     // dE/dB per spin in FP64 (the energy is the UMP2 expression with the
-    // SCF denominators), then w2 = sum_Q Lvv G^T - G^T Loo per spin
+    // SCF denominators), then w2 = sum_Q Lvv G^T - G^T Loo per spin, all on
+    // the device
     const int n = nbf_, Q = chol_->n_vectors();
     std::unique_ptr<device::Buffer> dBa, dBb;
     if (na_ * va_ > 0) dBa = std::make_unique<device::Buffer>((size_t)Q * na_ * va_);
@@ -1061,32 +1045,29 @@ void CholeskyOMP2::add_two_particle_gradient(const double* Ba, const double* Bb,
     std::unique_ptr<device::Buffer> Loo, Lvv;
     for (int s = 0; s < 2; ++s) {
         const int no = s == 0 ? na_ : nb_, nv = s == 0 ? va_ : vb_;
+        w2[s].reset();
         if (no * nv == 0) continue;
         if (s == 0 || !same_orbitals) {
-            const Mat& C = s == 0 ? scf_.C_alpha : scf_.C_beta;
-            auto dCo = to_device(left_cols(C, no));
-            auto dCv = to_device(right_cols(C, nv));
             Loo = std::make_unique<device::Buffer>((size_t)Q * no * no);
             Lvv = std::make_unique<device::Buffer>((size_t)Q * nv * nv);
-            device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, dCo->get(), no,
-                                              dCo->get(), no, MSHQC_PREC_FP64, Loo->get()),
+            device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, Co[s], no, Co[s], no,
+                                              MSHQC_PREC_FP64, Loo->get()),
                           "mshqc_hip_transform");
-            device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, dCv->get(), nv,
-                                              dCv->get(), nv, MSHQC_PREC_FP64, Lvv->get()),
+            device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, Cv[s], nv, Cv[s], nv,
+                                              MSHQC_PREC_FP64, Lvv->get()),
                           "mshqc_hip_transform");
         }
-        device::Buffer dw((size_t)nv * no);
+        w2[s] = std::make_unique<device::Buffer>((size_t)nv * no);
         device::check(mshqc_hip_omp2_grad2p(device::ctx(), (s == 0 ? dBa : dBb)->get(), Loo->get(), Lvv->get(), Q,
-                                            no, nv, dw.get()),
+                                            no, nv, w2[s]->get()),
                       "mshqc_hip_omp2_grad2p");
-        Mat w2(nv, no);
-        dw.download(w2.data(), w2.size());
-        w[s] = add(w[s], w2);
     }
 }
 
 OMP2Result CholeskyOMP2::compute() {
-    // cholesky_omp2.cc:427-513
+    // cholesky_omp2.cc:427-513, the iteration on the device: amplitudes, OPDM,
+    // generalised Fock, gradient and rotation; the host keeps the energies
+    // and the gradient norm for the convergence test
     const int n = nbf_, Q = chol_->n_vectors();
     if (chol_->n_basis() != n) throw std::runtime_error("CholeskyOMP2: Cholesky vectors do not match the basis");
     if (config_.print_level > 0)
@@ -1102,7 +1083,22 @@ OMP2Result CholeskyOMP2::compute() {
     auto devb = to_device(Vec(eb.begin() + nb_, eb.end()));
     device::Buffer dGoa((size_t)std::max(na_ * na_, 1)), dGva((size_t)std::max(va_ * va_, 1)),
         dGob((size_t)std::max(nb_ * nb_, 1)), dGvb((size_t)std::max(vb_ * vb_, 1));
-    device::Buffer dJ((size_t)n * n), dK((size_t)2 * n * n);
+    auto dH = to_device(H);
+    // per spin: C_occ [n][o], C_vir [n][v], the density (the SCF result's on
+    // the first iteration, :451-459) and the MO-basis generalised Fock
+    std::unique_ptr<device::Buffer> oCo[2], oCv[2], oP[2], oF[2], w2[2];
+    double *Co[2] = {nullptr, nullptr}, *Cv[2] = {nullptr, nullptr}, *P[2], *F[2];
+    for (int s = 0; s < 2; ++s) {
+        const int no = s == 0 ? na_ : nb_, nv = s == 0 ? va_ : vb_;
+        const Mat& C = s == 0 ? scf_.C_alpha : scf_.C_beta;
+        if (no > 0) Co[s] = (oCo[s] = to_device(left_cols(C, no)))->get();
+        if (nv > 0) Cv[s] = (oCv[s] = to_device(right_cols(C, nv)))->get();
+        P[s] = (oP[s] = to_device(s == 0 ? scf_.P_alpha : scf_.P_beta))->get();
+        F[s] = (oF[s] = std::make_unique<device::Buffer>((size_t)n * n))->get();
+    }
+    // beta follows alpha once the alpha-only rotation of na == nb has run
+    // (:484-490), or from the start when the SCF orbitals are the same
+    bool same_orbitals = na_ == nb_ && scf_.C_beta.a == scf_.C_alpha.a;
     double e_total_prev = scf_.energy_total, e_mp2 = 0.0, e_ss = 0.0, e_os = 0.0;
     bool converged = false;
     int iter = 0;
@@ -1110,9 +1106,18 @@ OMP2Result CholeskyOMP2::compute() {
         ++iter;
         // amplitudes, their energy and the OPDM blocks (:94-326)
         std::unique_ptr<device::Buffer> Ba, Bb;
-        if (na_ > 0 && va_ > 0) Ba = transform_device(*chol_, scf_.C_alpha, na_);
-        const bool same_orbitals = na_ == nb_ && scf_.C_beta.a == scf_.C_alpha.a;
-        if (nb_ > 0 && vb_ > 0 && !same_orbitals) Bb = transform_device(*chol_, scf_.C_beta, nb_);
+        if (na_ > 0 && va_ > 0) {
+            Ba = std::make_unique<device::Buffer>((size_t)Q * na_ * va_);
+            device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, Co[0], na_, Cv[0], va_,
+                                              MSHQC_PREC_FP64, Ba->get()),
+                          "mshqc_hip_transform");
+        }
+        if (nb_ > 0 && vb_ > 0 && !same_orbitals) {
+            Bb = std::make_unique<device::Buffer>((size_t)Q * nb_ * vb_);
+            device::check(mshqc_hip_transform(device::ctx(), chol_->device_vectors(), Q, n, Co[1], nb_, Cv[1], vb_,
+                                              MSHQC_PREC_FP64, Bb->get()),
+                          "mshqc_hip_transform");
+        }
         const double* pBb = same_orbitals ? (Ba ? Ba->get() : nullptr) : (Bb ? Bb->get() : nullptr);
         double E[3] = {0, 0, 0};
         device::check(mshqc_hip_omp2_opdm(device::ctx(), Ba ? Ba->get() : nullptr, pBb, Q, na_, va_, nb_, vb_,
@@ -1123,50 +1128,22 @@ OMP2Result CholeskyOMP2::compute() {
         e_os = E[2];
         e_mp2 = (E[0] + E[1]) + E[2];
         const double e_total_curr = scf_.energy_total + e_mp2;
-        // P_tot = P + C G C^T per spin (:451-459)
-        Mat Pt[2];
-        for (int s = 0; s < 2; ++s) {
-            const int no = s == 0 ? na_ : nb_, nv = s == 0 ? va_ : vb_;
-            const Mat& C = s == 0 ? scf_.C_alpha : scf_.C_beta;
-            Mat Goo(no, no), Gvv(nv, nv);
-            if (no > 0) (s == 0 ? dGoa : dGob).download(Goo.data(), Goo.size());
-            if (nv > 0) (s == 0 ? dGva : dGvb).download(Gvv.data(), Gvv.size());
-            Mat G(n, n);
-            for (int i = 0; i < no; ++i)
-                for (int j = 0; j < no; ++j) G(i, j) = Goo(i, j);
-            for (int a = 0; a < nv; ++a)
-                for (int b = 0; b < nv; ++b) G(no + a, no + b) = Gvv(a, b);
-            Pt[s] = add(s == 0 ? scf_.P_alpha : scf_.P_beta, matmul_nt(matmul(C, G), C));
-        }
-        // generalised Fock per spin: H + J[Pt_a + Pt_b] - K[Pt_s], in the MO
-        // basis (:331-369, :462-467; one J for both spins)
-        Mat P2(2 * n, n);
-        std::copy(Pt[0].a.begin(), Pt[0].a.end(), P2.a.begin());
-        std::copy(Pt[1].a.begin(), Pt[1].a.end(), P2.a.begin() + (size_t)n * n);
-        auto dP = to_device(P2);
-        device::check(mshqc_hip_jk(device::ctx(), chol_->device_vectors(), Q, n, dP->get(), 2, 0, dJ.get(), dK.get()),
-                      "mshqc_hip_jk");
-        Mat J(n, n), K(2 * n, n);
-        dJ.download(J.data(), J.size());
-        dK.download(K.data(), K.size());
-        Mat F_mo[2], w[2];
+        // P + C G C^T, then the generalised Fock H + J[Pt_a + Pt_b] - K[Pt_s]
+        // in the MO basis per spin (:331-369, :451-467)
+        device::check(mshqc_hip_omp2_fock(device::ctx(), chol_->device_vectors(), Q, n, dH->get(), Co[0], Cv[0],
+                                          na_, Co[1], Cv[1], nb_, P[0], P[1], dGoa.get(), dGva.get(), dGob.get(),
+                                          dGvb.get(), F[0], F[1]),
+                      "mshqc_hip_omp2_fock");
+        if (config_.two_particle_gradient)
+            two_particle_terms(Ba ? Ba->get() : nullptr, pBb, deoa->get(), deva->get(), deob->get(), devb->get(),
+                               same_orbitals, Co, Cv, w2);
         double g2 = 0.0;
         for (int s = 0; s < 2; ++s) {
-            Mat Ks(n, n);
-            std::copy(K.a.begin() + (size_t)s * n * n, K.a.begin() + (size_t)(s + 1) * n * n, Ks.a.begin());
-            const Mat F_ao = add(add(H, J), Ks, -1.0);
-            const Mat& C = s == 0 ? scf_.C_alpha : scf_.C_beta;
-            F_mo[s] = matmul(matmul_tn(C, F_ao), C);
-            const int no = s == 0 ? na_ : nb_, nv = s == 0 ? va_ : vb_;
-            w[s] = Mat(nv, no);
-            for (int a = 0; a < nv; ++a)
-                for (int i = 0; i < no; ++i) w[s](a, i) = 2.0 * F_mo[s](no + a, i);
-            g2 += dot(w[s], w[s]);
-        }
-        if (config_.two_particle_gradient) {
-            add_two_particle_gradient(Ba ? Ba->get() : nullptr, pBb, deoa->get(), deva->get(), deob->get(), devb->get(),
-                                      same_orbitals, w);
-            g2 = dot(w[0], w[0]) + dot(w[1], w[1]);
+            double gs = 0.0;
+            device::check(mshqc_hip_omp2_grad_norm(device::ctx(), n, s == 0 ? na_ : nb_, F[s],
+                                                   w2[s] ? w2[s]->get() : nullptr, &gs),
+                          "mshqc_hip_omp2_grad_norm");
+            g2 += gs;
         }
         const double g_norm = std::sqrt(g2);
         if (config_.print_level > 0)
@@ -1176,14 +1153,38 @@ OMP2Result CholeskyOMP2::compute() {
             converged = true;
             break;
         }
-        rotate(w[0], F_mo[0], na_, va_, scf_.C_alpha, scf_.P_alpha);
+        // damped, level-shifted rotation C <- C exp(-kappa) (:375-425)
+        device::check(mshqc_hip_omp2_rotate(device::ctx(), n, na_, F[0], w2[0] ? w2[0]->get() : nullptr, Co[0],
+                                            Cv[0], P[0]),
+                      "mshqc_hip_omp2_rotate");
         if (na_ != nb_) {
-            rotate(w[1], F_mo[1], nb_, vb_, scf_.C_beta, scf_.P_beta);
+            device::check(mshqc_hip_omp2_rotate(device::ctx(), n, nb_, F[1], w2[1] ? w2[1]->get() : nullptr, Co[1],
+                                                Cv[1], P[1]),
+                          "mshqc_hip_omp2_rotate");
         } else {
-            scf_.C_beta = scf_.C_alpha;
-            scf_.P_beta = scf_.P_alpha;
+            Co[1] = Co[0];
+            Cv[1] = Cv[0];
+            P[1] = P[0];
+            same_orbitals = true;
         }
         e_total_prev = e_total_curr;
+    }
+    // the final orbitals back on the host (columns: occupied, then virtual)
+    for (int s = 0; s < 2; ++s) {
+        const int no = s == 0 ? na_ : nb_, nv = s == 0 ? va_ : vb_;
+        Mat& C = s == 0 ? scf_.C_alpha : scf_.C_beta;
+        Mat Mo(n, no), Mv(n, nv);
+        if (no > 0) device::check(mshqc_hip_memcpy_d2h(device::ctx(), Mo.data(), Co[s], sizeof(double) * Mo.size()),
+                                  "download");
+        if (nv > 0) device::check(mshqc_hip_memcpy_d2h(device::ctx(), Mv.data(), Cv[s], sizeof(double) * Mv.size()),
+                                  "download");
+        for (int mu = 0; mu < n; ++mu) {
+            for (int i = 0; i < no; ++i) C(mu, i) = Mo(mu, i);
+            for (int a = 0; a < nv; ++a) C(mu, no + a) = Mv(mu, a);
+        }
+        device::check(mshqc_hip_memcpy_d2h(device::ctx(), (s == 0 ? scf_.P_alpha : scf_.P_beta).data(), P[s],
+                                           sizeof(double) * (size_t)n * n),
+                      "download");
     }
     OMP2Result res;
     res.energy_scf = scf_.energy_total;

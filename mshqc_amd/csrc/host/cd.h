@@ -425,10 +425,11 @@ private:
     const integrals::CholeskyERI* chol_ = nullptr;
     int nbf_, na_, nb_, va_, vb_;
 
-    void rotate(const Mat& w, const Mat& F_mo, int no, int nv, Mat& C, Mat& P) const;
-    // w[s] += the two-particle term of spin s (device B and orbital energies)
-    void add_two_particle_gradient(const double* Ba, const double* Bb, const double* eoa, const double* eva,
-                                   const double* eob, const double* evb, bool same_orbitals, Mat (&w)[2]) const;
+    // w2[s] = the two-particle term of spin s, on the device ([v][o]), from
+    // device B, orbital energies and orbital blocks
+    void two_particle_terms(const double* Ba, const double* Bb, const double* eoa, const double* eva,
+                            const double* eob, const double* evb, bool same_orbitals, double* const (&Co)[2],
+                            double* const (&Cv)[2], std::unique_ptr<device::Buffer> (&w2)[2]) const;
 };
 
 }  // namespace mshqc

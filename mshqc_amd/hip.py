@@ -46,6 +46,9 @@ _SIGS = {
     "mshqc_hip_omp2_opdm": [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp, _vp, _vp,
                             _D, _vp, _vp, _vp, _vp],
     "mshqc_hip_omp2_grad2p": [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _vp],
+    "mshqc_hip_omp2_fock": [_vp, _vp, _c_int, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _c_int] + [_vp] * 8,
+    "mshqc_hip_omp2_grad_norm": [_vp, _c_int, _c_int, _vp, _vp, _D],
+    "mshqc_hip_omp2_rotate": [_vp, _c_int, _c_int, _vp, _vp, _vp, _vp, _vp],
     "mshqc_hip_transform": [_vp, _vp, _c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _vp],
     "mshqc_hip_rmp2_energy": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D],
     "mshqc_hip_rmp2_grad": [_vp, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_int, _D,
@@ -371,6 +374,39 @@ class Context:
                                          self._ptr(Lvv, (Q, v, v), "Lvv"), Q, o, v, self._ptr(w2))
         self._check(rc, "mshqc_hip_omp2_grad2p")
         return w2
+
+    def omp2_fock(self, L, H, Co_a, Cv_a, Co_b, Cv_b, P_a, P_b, Goo_a, Gvv_a, Goo_b, Gvv_b):
+        """MO-basis generalised Fock (F_a, F_b) [n, n] of one OMP2 iteration."""
+        torch = self._torch
+        Q, n, _ = L.shape
+        na, nb = Co_a.shape[1], Co_b.shape[1]
+        F = [torch.empty((n, n), dtype=torch.float64, device=L.device) for _ in range(2)]
+        self._follow_stream()
+        p = self._ptr
+        rc = lib().mshqc_hip_omp2_fock(self.h, p(L, name="L"), Q, n, p(H, (n, n), "H"), p(Co_a), p(Cv_a), na,
+                                       p(Co_b), p(Cv_b), nb, p(P_a, (n, n)), p(P_b, (n, n)), p(Goo_a), p(Gvv_a),
+                                       p(Goo_b), p(Gvv_b), p(F[0]), p(F[1]))
+        self._check(rc, "mshqc_hip_omp2_fock")
+        return tuple(F)
+
+    def omp2_grad_norm(self, F, o, w2=None) -> float:
+        n = F.shape[0]
+        g2 = ctypes.c_double()
+        self._follow_stream()
+        rc = lib().mshqc_hip_omp2_grad_norm(self.h, n, int(o), self._ptr(F, (n, n), "F"),
+                                            self._ptr(w2) if w2 is not None else None, ctypes.byref(g2))
+        self._check(rc, "mshqc_hip_omp2_grad_norm")
+        return g2.value
+
+    def omp2_rotate(self, F, Co, Cv, P, w2=None):
+        """One OMP2 orbital step in place on Co [n, o], Cv [n, v], P [n, n]."""
+        n = F.shape[0]
+        o = Co.shape[1]
+        self._follow_stream()
+        rc = lib().mshqc_hip_omp2_rotate(self.h, n, o, self._ptr(F, (n, n), "F"),
+                                         self._ptr(w2) if w2 is not None else None, self._ptr(Co, (n, o), "Co"),
+                                         self._ptr(Cv, (n, n - o), "Cv"), self._ptr(P, (n, n), "P"))
+        self._check(rc, "mshqc_hip_omp2_rotate")
 
     def rmp3(self, B, Loo, Lvv, t1, eps_o, eps_v, want_t2=False):
         """(E3, t2_2 or None) of CD-RMP3 from B [Q,o,v], Loo [Q,o,o], Lvv [Q,v,v], t1 [o,o,v,v]."""

@@ -188,3 +188,53 @@ def test_cd_omp2_two_particle(m, mult):
         cfg.gradient_threshold = 1e-7
     res = _compare(m, mol, b, eng, chol, L, H, scf, cfg)
     assert res.converged
+
+
+def test_device_orbital_step_matches_oracle(ctx):
+    """mshqc_hip_omp2_fock / _grad_norm / _rotate (cholesky_omp2.cc:331-425 on
+    the device) against numpy: P + C G C^T, the generalised Fock in the MO
+    basis from the oracle's J/K, the gradient norm, and the oracle's step
+    C exp(-kappa) (omp2_ref.rotation_step, scipy's expm), with and without a
+    two-particle term."""
+    import scipy.linalg as sl
+    import torch
+
+    import oracle
+
+    rng = np.random.default_rng(97)
+    n, Q, na, nb = 24, 40, 5, 3
+    L = syn.gen_L(Q, n, rng)
+    A = rng.standard_normal((n, n))
+    H = 0.5 * (A + A.T) - 3.0 * np.eye(n)
+    _, U = np.linalg.eigh(H)
+    C = [U @ sl.expm(0.1 * (lambda X: X - X.T)(rng.standard_normal((n, n)))) for _ in range(2)]
+    no = (na, nb)
+    P = [C[s][:, :no[s]] @ C[s][:, :no[s]].T + 0.01 * np.eye(n) for s in range(2)]
+    G = []
+    for s in range(2):
+        X = rng.standard_normal((no[s], no[s]))
+        Y = rng.standard_normal((n - no[s], n - no[s]))
+        G += [-0.01 * (X @ X.T), 0.01 * (Y @ Y.T)]
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    Co = [dev(C[s][:, :no[s]]) for s in range(2)]
+    Cv = [dev(C[s][:, no[s]:]) for s in range(2)]
+    Fd = ctx.omp2_fock(dev(L), dev(H), Co[0], Cv[0], Co[1], Cv[1], dev(P[0]), dev(P[1]), *[dev(g) for g in G])
+    Pt = []
+    for s in range(2):
+        Gf = np.zeros((n, n))
+        Gf[:no[s], :no[s]], Gf[no[s]:, no[s]:] = G[2 * s], G[2 * s + 1]
+        Pt.append(P[s] + C[s] @ Gf @ C[s].T)
+    J, K = oracle.jk(L, np.stack(Pt))
+    for s in range(2):
+        F_ref = C[s].T @ (H + J - K[s]) @ C[s]
+        check_close(to_np(Fd[s]), F_ref, 1e-12, f"F spin {s}")
+        w2 = 0.05 * rng.standard_normal((n - no[s], no[s])) if s == 1 else None
+        w = 2.0 * F_ref[no[s]:, :no[s]] + (0.0 if w2 is None else w2)
+        g2 = ctx.omp2_grad_norm(Fd[s], no[s], None if w2 is None else dev(w2))
+        assert abs(g2 - np.sum(w * w)) <= 1e-12 * np.sum(w * w)
+        Cn = C[s] @ sl.expm(-omp2_ref.rotation_step(w, F_ref, no[s]))
+        Pd = dev(P[s])
+        ctx.omp2_rotate(Fd[s], Co[s], Cv[s], Pd, None if w2 is None else dev(w2))
+        check_close(to_np(Co[s]), Cn[:, :no[s]], 1e-12, f"C_occ spin {s}")
+        check_close(to_np(Cv[s]), Cn[:, no[s]:], 1e-12, f"C_vir spin {s}")
+        check_close(to_np(Pd), Cn[:, :no[s]] @ Cn[:, :no[s]].T, 1e-12, f"P spin {s}")

@@ -30,12 +30,16 @@ constexpr int KS = 16;    // Q per pair-kernel stage (one 32x32x16 MFMA K step)
 constexpr int RB = 32;    // LDS bytes per panel row, slice and pair-kernel stage
 constexpr int KC = 32;    // contraction chunk of the gradient GEMM (one 16x16x32 MFMA K step)
 constexpr int LROW = 64;  // LDS bytes per 32-element BF16 row
-// Exponent block of the gradient-GEMM operands along k.  Its tier-0 sums
-// reach 2^24, where the MFMA accumulation can lose a last bit (2^-24 of the
-// block sum); with three slices the truncation sits near 2^-26, so the
-// 256-wide block (half the FP64 flushes of 128) keeps the gradient well
-// inside its bar.
-constexpr int GB = 256;
+// Exponent block widths of the BF16 GEMM operands along k (the GEMM flushes
+// its FP32 class sums to FP64 once per block).  The gradient contractions
+// use 512: their tier-0 sums are sums of products of unrelated operands
+// (B against amplitudes), random in sign, so they stay far below 2^24, where
+// an MFMA accumulation can lose a last bit; the FP64 flushes halve against
+// 256-wide blocks (4% of the GEMM time, gradient error 3.5e-8 -> 4.8e-8 at
+// o=100/v=1100/Q=4000).  The transforms keep 256, which pads their AO index
+// less (n = 1200: 1280 instead of 1536).
+constexpr int GB_GRAD = 512;
+constexpr int GB_TRANSFORM = 256;
 constexpr int GT = 128;   // gradient-GEMM workgroup tile (rows and columns)
 constexpr int GW = 8;     // gradient-GEMM waves per workgroup (two per SIMD, 64 x 32 each)
 
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(256) void self_terms_kernel(const double* __restric
 // ex[nseg*(vp/GB)][ldx] (k-block major, ldx >= R rows: four consecutive rows
 // are one dword).  One wave per (row, seg, GB-wide b tile); each lane slices
 // GB/64 elements.
-template <int S>
+template <int S, int GB>
 __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restrict__ src, long ld, int R,
                                                          int nseg, int v, int vp,
                                                          uint16_t* __restrict__ dst,
@@ -193,16 +197,17 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(const double* __restric
 // Transposed slicing of Z[(i',a)][(j',b)] (rows i'*v + a, i' < nI; ld = ldz;
 // nc = nJ*vr columns):  dst[S][nc][nI*vp] with row n = column of Z and
 // column i'*vp + a;  exponents ex[nI*(vp/GB)][ldx] (k-block major, ldx >= nc).
-// A workgroup transposes a tile of SC_COLS columns x GB values of a through
+// A workgroup transposes a tile of sc_cols(GB) columns x GB values of a through
 // LDS.
-constexpr int SC_COLS = 16;
+// (16 columns for 256-wide blocks, 8 for 512: the LDS tile stays 37 KiB or less)
+constexpr int sc_cols(int GB) { return GB >= 512 ? 8 : 16; }
 
-template <int S>
+template <int S, int GB>
 __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restrict__ Z, long ldz, int nI,
                                                          int v, int vp, int nc,
                                                          uint16_t* __restrict__ dst,
                                                          int8_t* __restrict__ ex, long ldx) {
-    constexpr int NC = SC_COLS, PARTS = 256 / NC, PER = GB / PARTS;  // PER values of a per thread
+    constexpr int NC = sc_cols(GB), PARTS = 256 / NC, PER = GB / PARTS;  // PER values of a per thread
     static_assert(PER % 8 == 0, "whole 16-byte stores");
     __shared__ double T[GB][NC + 1];
     __shared__ int exs[NC];
@@ -829,7 +834,7 @@ struct OutRows {
 // fit 160 KiB, else two (loads one stage ahead).
 constexpr int gemm_nbuf(int S) { return 3 * 2 * S * GT * LROW <= 160 * 1024 ? 3 : 2; }
 
-template <int S>
+template <int S, int GB>
 __global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, SlicedMat X, int K,
                                                                double* __restrict__ C, OutRows out) {
     constexpr long PANEL = (long)GT * LROW;   // bytes of one slice panel (128 rows x 32 k)
@@ -1016,12 +1021,20 @@ struct SlicePair {
     }
 };
 
+// The functors below take the exponent block width gb (GB_GRAD or
+// GB_TRANSFORM) at run time and launch the matching instantiation.
 template <int S>
 struct SliceRows {
-    static int run(mshqc_hip_ctx* ctx, const double* src, long ld, int R, int nseg, int v, int vp,
-                   uint16_t* dst, int8_t* ex, long ldx) {
+    template <int GB>
+    static void launch(mshqc_hip_ctx* ctx, const double* src, long ld, int R, int nseg, int v, int vp,
+                       uint16_t* dst, int8_t* ex, long ldx) {
         const long jobs = (long)R * nseg * (vp / GB);
-        slice_rows_kernel<S><<<ceil_div(jobs, 4), 256, 0, ctx->stream>>>(src, ld, R, nseg, v, vp, dst, ex, ldx);
+        slice_rows_kernel<S, GB><<<ceil_div(jobs, 4), 256, 0, ctx->stream>>>(src, ld, R, nseg, v, vp, dst, ex, ldx);
+    }
+    static int run(mshqc_hip_ctx* ctx, int gb, const double* src, long ld, int R, int nseg, int v, int vp,
+                   uint16_t* dst, int8_t* ex, long ldx) {
+        if (gb == GB_GRAD) launch<GB_GRAD>(ctx, src, ld, R, nseg, v, vp, dst, ex, ldx);
+        else launch<GB_TRANSFORM>(ctx, src, ld, R, nseg, v, vp, dst, ex, ldx);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -1029,10 +1042,16 @@ struct SliceRows {
 
 template <int S>
 struct SliceCols {
-    static int run(mshqc_hip_ctx* ctx, const double* Z, long ldz, int nI, int v, int vp, int nc,
+    template <int GB>
+    static void launch(mshqc_hip_ctx* ctx, const double* Z, long ldz, int nI, int v, int vp, int nc,
+                       uint16_t* dst, int8_t* ex, long ldx) {
+        dim3 grid(ceil_div(nc, sc_cols(GB)), nI * (vp / GB));
+        slice_cols_kernel<S, GB><<<grid, 256, 0, ctx->stream>>>(Z, ldz, nI, v, vp, nc, dst, ex, ldx);
+    }
+    static int run(mshqc_hip_ctx* ctx, int gb, const double* Z, long ldz, int nI, int v, int vp, int nc,
                    uint16_t* dst, int8_t* ex, long ldx) {
-        dim3 grid(ceil_div(nc, SC_COLS), nI * (vp / GB));
-        slice_cols_kernel<S><<<grid, 256, 0, ctx->stream>>>(Z, ldz, nI, v, vp, nc, dst, ex, ldx);
+        if (gb == GB_GRAD) launch<GB_GRAD>(ctx, Z, ldz, nI, v, vp, nc, dst, ex, ldx);
+        else launch<GB_TRANSFORM>(ctx, Z, ldz, nI, v, vp, nc, dst, ex, ldx);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -1079,12 +1098,16 @@ struct PairLaunch {
 
 template <int S>
 struct GemmLaunch {
-    static int run(mshqc_hip_ctx* ctx, const SlicedMat& A, const SlicedMat& X, int K, double* C, long ldc,
+    static int run(mshqc_hip_ctx* ctx, int gb, const SlicedMat& A, const SlicedMat& X, int K, double* C, long ldc,
                    int mdiv = 0, long ld1 = 0) {
-        if (K % GB != 0)
-            return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 GEMM: K=%d is not a multiple of %d", K, GB);
+        if (K % gb != 0)
+            return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 GEMM: K=%d is not a multiple of %d", K, gb);
         dim3 grid(grouped_grid(ceil_div(A.rows, GT), ceil_div(X.rows, GT)));
-        gemm_bf16_nt_kernel<S><<<grid, GW * 64, gemm_lds(S), ctx->stream>>>(A, X, K, C, OutRows{ldc, mdiv, ld1});
+        const OutRows out{ldc, mdiv, ld1};
+        if (gb == GB_GRAD)
+            gemm_bf16_nt_kernel<S, GB_GRAD><<<grid, GW * 64, gemm_lds(S), ctx->stream>>>(A, X, K, C, out);
+        else
+            gemm_bf16_nt_kernel<S, GB_TRANSFORM><<<grid, GW * 64, gemm_lds(S), ctx->stream>>>(A, X, K, C, out);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -1111,7 +1134,9 @@ struct AllowLds {
 template <int S>
 struct AllowLdsGemm {
     static int run(mshqc_hip_ctx* ctx) {
-        hipFuncSetAttribute((const void*)gemm_bf16_nt_kernel<S>,
+        hipFuncSetAttribute((const void*)gemm_bf16_nt_kernel<S, GB_GRAD>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)gemm_lds(S));
+        hipFuncSetAttribute((const void*)gemm_bf16_nt_kernel<S, GB_TRANSFORM>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)gemm_lds(S));
         hipGetLastError();
         (void)ctx;
@@ -1175,12 +1200,12 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
         const int SG = o.gslices;
         o.bol = std::max(1, std::min(bo_l, ol));
         o.bor = Br ? std::max(1, std::min(bo_r, orr)) : o.bol;
-        o.kwbl = round_up(o.bol * vl, GB);
-        o.kwbr = Br ? round_up(o.bor * vr, GB) : o.kwbl;
+        o.kwbl = round_up(o.bol * vl, GB_GRAD);
+        o.kwbr = Br ? round_up(o.bor * vr, GB_GRAD) : o.kwbl;
         const int nbl = ceil_div(ol, o.bol), nbr = Br ? ceil_div(orr, o.bor) : 0;
         const size_t gl = (size_t)nbl * SG * Q * o.kwbl * 2, gr = (size_t)nbr * SG * Q * o.kwbr * 2;
         const long Qx = round_up(Q, GT);  // exponent tables: [kb][Qx] per block
-        const size_t gel = (size_t)nbl * Qx * (o.kwbl / GB), ger = (size_t)nbr * Qx * (o.kwbr / GB);
+        const size_t gel = (size_t)nbl * Qx * (o.kwbl / GB_GRAD), ger = (size_t)nbr * Qx * (o.kwbr / GB_GRAD);
         char* g = static_cast<char*>(workspace(ctx, WS_SLICE2, gl + gr + gel + ger + 1024));
         if (!g) return MSHQC_HIP_ERR_ALLOC;
         o.gl = reinterpret_cast<uint16_t*>(g);
@@ -1195,9 +1220,9 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
             int8_t* ex = const_cast<int8_t*>(side ? o.ger : o.gel);
             for (int b = 0, i0 = 0; i0 < oo; ++b, i0 += bo) {
                 const int nb = std::min(bo, oo - i0);
-                rc = by_gslices<SliceRows>(SG, ctx, src + (long)i0 * vv, (long)oo * vv, Q, 1, nb * vv,
-                                          round_up(nb * vv, GB), dst + (size_t)b * SG * Q * kwb,
-                                          ex + (size_t)b * Qx * (kwb / GB), Qx);
+                rc = by_gslices<SliceRows>(SG, ctx, GB_GRAD, src + (long)i0 * vv, (long)oo * vv, Q, 1, nb * vv,
+                                          round_up(nb * vv, GB_GRAD), dst + (size_t)b * SG * Q * kwb,
+                                          ex + (size_t)b * Qx * (kwb / GB_GRAD), Qx);
                 if (rc) return rc;
             }
         }
@@ -1219,6 +1244,7 @@ int bf16_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const doub
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     // Contraction width (an AO index) padded to whole exponent blocks; every
     // operand is sliced with one exponent per row and GB-wide k block.
+    const int GB = GB_TRANSFORM;
     const long kw = round_up(n, GB), nkb = kw / GB;
     const long ox = round_up(o, GT), vx = round_up(v, GT);
     // Q chunk: about 1.5 GiB of L slices at a time.
@@ -1242,9 +1268,9 @@ int bf16_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const doub
     double* T = reinterpret_cast<double*>(w + o_t);
     // C_occ^T: rows i, k = nu;  C_vir^T: rows a, k = mu (transposed slicing
     // of the n x o and n x v coefficient blocks)
-    int rc = by_tslices<SliceCols>(S, ctx, Cocc, (long)o, 1, n, (int)kw, o, u16(o_cs), i8(o_cx), ox);
+    int rc = by_tslices<SliceCols>(S, ctx, GB, Cocc, (long)o, 1, n, (int)kw, o, u16(o_cs), i8(o_cx), ox);
     if (rc) return rc;
-    rc = by_tslices<SliceCols>(S, ctx, Cvir, (long)v, 1, n, (int)kw, v, u16(o_vs), i8(o_vx), vx);
+    rc = by_tslices<SliceCols>(S, ctx, GB, Cvir, (long)v, 1, n, (int)kw, v, u16(o_vs), i8(o_vx), vx);
     if (rc) return rc;
     const SlicedMat Ct{u16(o_cs), (long)o * kw, kw, i8(o_cx), ox, o};
     const SlicedMat Vt{u16(o_vs), (long)v * kw, kw, i8(o_vx), vx, v};
@@ -1252,7 +1278,7 @@ int bf16_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const doub
         const int nq = std::min(qc, Q - q0);
         const long nl = (long)nq * n, nt = (long)o * nq;
         // L rows (q, mu), k = nu
-        rc = by_tslices<SliceRows>(S, ctx, L + (long)q0 * n * n, (long)n, (int)nl, 1, n, (int)kw, u16(o_ls),
+        rc = by_tslices<SliceRows>(S, ctx, GB, L + (long)q0 * n * n, (long)n, (int)nl, 1, n, (int)kw, u16(o_ls),
                                    i8(o_lx), rlx);
         if (rc) return rc;
         // T[i][(q, mu)] = sum_nu C_occ[nu][i] L^q[mu][nu]  (= (L^q C_occ)[mu][i], L^q symmetric)
@@ -1260,11 +1286,11 @@ int bf16_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const doub
         const SlicedMat Ls{u16(o_ls), nl * kw, kw, i8(o_lx), rlx, (int)nl};
         {
             ProfScope ps(ctx, PROF_GEMM);
-            rc = by_tslices<GemmLaunch>(S, ctx, Ct, Ls, (int)kw, T, nl, 0, 0L);
+            rc = by_tslices<GemmLaunch>(S, ctx, GB, Ct, Ls, (int)kw, T, nl, 0, 0L);
         }
         if (rc) return rc;
         // T rows (i, q) = row i*nq + q, k = mu
-        rc = by_tslices<SliceRows>(S, ctx, T, (long)n, (int)nt, 1, n, (int)kw, u16(o_ts), i8(o_tx), rtx);
+        rc = by_tslices<SliceRows>(S, ctx, GB, T, (long)n, (int)nt, 1, n, (int)kw, u16(o_ts), i8(o_tx), rtx);
         if (rc) return rc;
         // B[q][i][a] = sum_mu T[(i, q)][mu] C_vir[mu][a]: GEMM row m = i*nq + q
         // is stored at (m / nq) * v + (m % nq) * o*v
@@ -1273,7 +1299,7 @@ int bf16_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const doub
         const SlicedMat Ts{u16(o_ts), nt * kw, kw, i8(o_tx), rtx, (int)nt};
         {
             ProfScope ps(ctx, PROF_GEMM);
-            rc = by_tslices<GemmLaunch>(S, ctx, Ts, Vt, (int)kw, Bq, (long)o * v, nq, (long)v);
+            rc = by_tslices<GemmLaunch>(S, ctx, GB, Ts, Vt, (int)kw, Bq, (long)o * v, nq, (long)v);
         }
         if (rc) return rc;
     }
@@ -1295,36 +1321,36 @@ int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& o, const double* 
     int rc;
     if (dBl) {
         // dBl[q][(i,a)] += sum_{(j,b)} Br[q][(j,b)] Z[(i,a)][(j,b)]
-        const long R = (long)nI * vl, kw = round_up(nJ * vr, GB), Rx = round_up(R, GT), Qx = round_up(Q, GT);
-        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)Rx * (kw / GB);
+        const long R = (long)nI * vl, kw = round_up(nJ * vr, GB_GRAD), Rx = round_up(R, GT), Qx = round_up(Q, GT);
+        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)Rx * (kw / GB_GRAD);
         char* w = static_cast<char*>(workspace(ctx, WS_ZS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;
         uint16_t* zs = reinterpret_cast<uint16_t*>(w);
         int8_t* zx = reinterpret_cast<int8_t*>(w + zb);
-        rc = by_gslices<SliceRows>(S, ctx, Z, ldz, (int)R, 1, nJ * vr, (int)kw, zs, zx, Rx);
+        rc = by_gslices<SliceRows>(S, ctx, GB_GRAD, Z, ldz, (int)R, 1, nJ * vr, (int)kw, zs, zx, Rx);
         if (rc) return rc;
         const long b = J0 / o.bor;
-        SlicedMat A{o.gr + (size_t)b * S * Q * o.kwbr, (long)Q * kw, kw, o.ger + (size_t)b * Qx * (o.kwbr / GB),
+        SlicedMat A{o.gr + (size_t)b * S * Q * o.kwbr, (long)Q * kw, kw, o.ger + (size_t)b * Qx * (o.kwbr / GB_GRAD),
                     Qx, Q};
         SlicedMat X{zs, (long)R * kw, kw, zx, Rx, (int)R};
-        rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBl + (long)I0 * vl, (long)o.ol * vl, 0, 0L);
+        rc = by_gslices<GemmLaunch>(S, ctx, GB_GRAD, A, X, (int)kw, dBl + (long)I0 * vl, (long)o.ol * vl, 0, 0L);
         if (rc) return rc;
     }
     if (second && dBr) {
         // dBr[q][(j,b)] += sum_{(i,a)} Bl[q][(i,a)] Z[(i,a)][(j,b)]
-        const long R = (long)nJ * vr, kw = round_up(nI * vl, GB), Rx = round_up(R, GT), Qx = round_up(Q, GT);
-        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)Rx * (kw / GB);
+        const long R = (long)nJ * vr, kw = round_up(nI * vl, GB_GRAD), Rx = round_up(R, GT), Qx = round_up(Q, GT);
+        const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)Rx * (kw / GB_GRAD);
         char* w = static_cast<char*>(workspace(ctx, WS_ZTS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;
         uint16_t* zs = reinterpret_cast<uint16_t*>(w);
         int8_t* zx = reinterpret_cast<int8_t*>(w + zb);
-        rc = by_gslices<SliceCols>(S, ctx, Z, ldz, 1, nI * vl, (int)kw, (int)R, zs, zx, Rx);
+        rc = by_gslices<SliceCols>(S, ctx, GB_GRAD, Z, ldz, 1, nI * vl, (int)kw, (int)R, zs, zx, Rx);
         if (rc) return rc;
         const long b = I0 / o.bol;
-        SlicedMat A{o.gl + (size_t)b * S * Q * o.kwbl, (long)Q * kw, kw, o.gel + (size_t)b * Qx * (o.kwbl / GB),
+        SlicedMat A{o.gl + (size_t)b * S * Q * o.kwbl, (long)Q * kw, kw, o.gel + (size_t)b * Qx * (o.kwbl / GB_GRAD),
                     Qx, Q};
         SlicedMat X{zs, (long)R * kw, kw, zx, Rx, (int)R};
-        rc = by_gslices<GemmLaunch>(S, ctx, A, X, (int)kw, dBr + (long)J0 * vr, (long)o.orr * vr, 0, 0L);
+        rc = by_gslices<GemmLaunch>(S, ctx, GB_GRAD, A, X, (int)kw, dBr + (long)J0 * vr, (long)o.orr * vr, 0, 0L);
         if (rc) return rc;
     }
     return MSHQC_HIP_OK;

@@ -17,9 +17,11 @@
 //   tier 1 (c = 1): multiples of 2^-9, <= 2^7 each, sums <= 2^15, i.e. at
 //                   most 2^24 units                                     -> exact
 // (with 256-wide blocks, tier-0 sums up to 2^24 measurably lost their last
-// bit on gfx950 MFMAs, so the pair kernel's blocks stay at 128; the
-// three-slice gradient GEMM, whose truncation sits near 2^-26, uses
-// 256-wide k blocks and two tiers, c = 0 and c >= 1)
+// bit on gfx950 MFMAs, so the pair kernel's blocks stay at 128, where the
+// coherent self-term sums reach that range; the three-slice gradient GEMM,
+// whose tier-0 sums are random in sign and whose truncation sits near
+// 2^-26, uses 512-wide k blocks (the transforms 256) and two tiers, c = 0
+// and c >= 1)
 //   tier 2 (c >= 2): below 2^-2 per product; FP32 rounding there sits near
 //                    2^-42 of the tier-0 scale, under the truncation error.
 // After every block the tiers are added in FP64 and scaled by 2^{eA+eB-16}.

@@ -13,6 +13,7 @@
 //   gemm_bf16_nt_kernel C[m][n] += sum_k A[m][k] X[n][k] from slices
 // The arithmetic contract is described in ozaki.h.
 #include <algorithm>
+#include <type_traits>
 
 #include "mp2_bf16.h"
 #include "mp2_internal.h"
@@ -21,12 +22,22 @@
 
 namespace {
 
-// Q block of the (ia|jb) kernel: exponent block and exact-FP32 span.  Tier-0
-// class sums stay below 2^23 at 128; at 256 (sums up to 2^24) the energy and
-// gradient errors measured on gfx950 grew ~3x and ~12x, so the MFMA
-// accumulation is not exact in that last bit.
-constexpr int QB = 128;
 constexpr int KS = 16;    // Q per pair-kernel stage (one 32x32x16 MFMA K step)
+// Q block of the (ia|jb) kernel (exponent block and exact-FP32 span): SPB
+// stages of KS, 8 or 10, whichever pads Q to fewer stages (Q = 1600, 2400,
+// 4000, 5600 are whole 160-blocks; 128-blocks left 2-6 stages of padding).
+// Tier-0 class sums stay below 160 * 2^16 < 2^24; at 256 Q (sums up to
+// 2^24) the energy and gradient errors measured on gfx950 grew ~3x and ~12x,
+// so the MFMA accumulation is not exact in that last bit.
+// Five slices keep 8: their 10-stage symmetric kernels do not fit the
+// register file without spilling.
+constexpr int SPB_SHORT = 8, SPB_LONG = 10;
+constexpr bool spb_long_ok(int S) { return S != 5; }
+int pair_stages_per_block(int Q, int S) {
+    const int st = (Q + KS - 1) / KS;
+    auto padded = [st](int spb) { return (st + spb - 1) / spb * spb; };
+    return spb_long_ok(S) && padded(SPB_LONG) < padded(SPB_SHORT) ? SPB_LONG : SPB_SHORT;
+}
 constexpr int RB = 32;    // LDS bytes per panel row, slice and pair-kernel stage
 constexpr int KC = 32;    // contraction chunk of the gradient GEMM (one 16x16x32 MFMA K step)
 constexpr int LROW = 64;  // LDS bytes per 32-element BF16 row
@@ -73,9 +84,9 @@ inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 // caller zeroes Pe, so the padding a >= v reads 0).  One workgroup per 32
 // rows x QB q; the tile is transposed through LDS (dynamic, QB x 33 doubles)
 // so both the FP64 reads and the BF16 writes are contiguous.
-constexpr size_t slice_pair_lds() { return (size_t)QB * 33 * sizeof(double); }
+constexpr size_t slice_pair_lds(int QB) { return (size_t)QB * 33 * sizeof(double); }
 
-template <int S>
+template <int S, int QB>
 __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restrict__ B, int Q, int ov, int v,
                                                          int vE, int o, int Qp, uint16_t* __restrict__ Ps,
                                                          int8_t* __restrict__ Pe) {
@@ -439,7 +450,7 @@ __device__ __forceinline__ void pair_epilogue32(const PairArgs& p, int i, int j,
     pair_epilogue32_r<MODE>(p, p.evl, p.evr, p.Z, i, j, w, a0, b0, mirror, lane, G, H, slot);
 }
 
-template <int MODE, int S>
+template <int MODE, int S, int SPB>
 __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16_t* __restrict__ Psl,
                                                         const int8_t* __restrict__ Pel,
                                                         const uint16_t* __restrict__ Psr,
@@ -599,7 +610,7 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 #pragma unroll
             for (int u = 0; u < 4; ++u) acc64[s][t][u] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    constexpr int SPB = QB / KS;  // stages per exponent block
+    constexpr int QB = SPB * KS;  // Q per exponent block
 #pragma unroll
     for (int sl = 0; sl < S; ++sl) issue_slice(0, sl, lds + cur[sl]);
 #pragma unroll
@@ -1011,11 +1022,17 @@ constexpr size_t gemm_lds(int S) { return (size_t)gemm_nbuf(S) * 2 * S * GT * LR
 
 template <int S>
 struct SlicePair {
-    static int run(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v, int vE, int Qp, uint16_t* Ps,
-                   int8_t* Pe) {
+    template <int QB>
+    static void launch(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v, int vE, int Qp, uint16_t* Ps,
+                       int8_t* Pe) {
         const int ov = o * v;
         dim3 grid(ceil_div(ov, 32), Qp / QB);
-        slice_pair_kernel<S><<<grid, 256, slice_pair_lds(), ctx->stream>>>(B, Q, ov, v, vE, o, Qp, Ps, Pe);
+        slice_pair_kernel<S, QB><<<grid, 256, slice_pair_lds(QB), ctx->stream>>>(B, Q, ov, v, vE, o, Qp, Ps, Pe);
+    }
+    static int run(mshqc_hip_ctx* ctx, int spb, const double* B, int Q, int o, int v, int vE, int Qp,
+                   uint16_t* Ps, int8_t* Pe) {
+        if (spb == SPB_LONG) launch<SPB_LONG * KS>(ctx, B, Q, o, v, vE, Qp, Ps, Pe);
+        else launch<SPB_SHORT * KS>(ctx, B, Q, o, v, vE, Qp, Ps, Pe);
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -1075,22 +1092,28 @@ struct PairLaunch {
         dim3 grid((unsigned)nwg);
         const uint16_t* sr = o.sr ? o.sr : o.sl;
         const int8_t* xr = o.sr ? o.xr : o.xl;
-        switch (mode) {
-            case MODE_RMP2:
-                pair_bf16_kernel<MODE_RMP2, S><<<grid, 256, pair_lds(S, true), ctx->stream>>>(
+        if (mode != MODE_RMP2 && mode != MODE_SS && mode != MODE_OS)
+            return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 pair kernel: unsupported mode %d", mode);
+        auto go = [&](auto spb_c) {
+            constexpr int SPB = decltype(spb_c)::value;
+            if (mode == MODE_RMP2)
+                pair_bf16_kernel<MODE_RMP2, S, SPB><<<grid, 256, pair_lds(S, true), ctx->stream>>>(
                     a, o.sl, o.xl, sr, xr, o.Qp, o.vEl, o.vEr);
-                break;
-            case MODE_SS:
-                pair_bf16_kernel<MODE_SS, S><<<grid, 256, pair_lds(S, true), ctx->stream>>>(
+            else if (mode == MODE_SS)
+                pair_bf16_kernel<MODE_SS, S, SPB><<<grid, 256, pair_lds(S, true), ctx->stream>>>(
                     a, o.sl, o.xl, sr, xr, o.Qp, o.vEl, o.vEr);
-                break;
-            case MODE_OS:
-                pair_bf16_kernel<MODE_OS, S><<<grid, 256, pair_lds(S, false), ctx->stream>>>(
+            else
+                pair_bf16_kernel<MODE_OS, S, SPB><<<grid, 256, pair_lds(S, false), ctx->stream>>>(
                     a, o.sl, o.xl, sr, xr, o.Qp, o.vEl, o.vEr);
-                break;
-            default:
-                return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 pair kernel: unsupported mode %d", mode);
+        };
+        if constexpr (spb_long_ok(S)) {
+            if (o.spb == SPB_LONG) {
+                go(std::integral_constant<int, SPB_LONG>{});
+                LAUNCH_CHECK(ctx);
+                return 0;
+            }
         }
+        go(std::integral_constant<int, SPB_SHORT>{});
         LAUNCH_CHECK(ctx);
         return 0;
     }
@@ -1115,16 +1138,23 @@ struct GemmLaunch {
 
 template <int S>
 struct AllowLds {
+    template <int SPB>
+    static void allow() {
+        hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_RMP2, S, SPB>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, true));
+        hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_SS, S, SPB>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, true));
+        hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_OS, S, SPB>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, false));
+    }
     static int run(mshqc_hip_ctx* ctx) {
         // Dynamic LDS above 64 KiB must be opted into per kernel.
-        hipFuncSetAttribute((const void*)slice_pair_kernel<S>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)slice_pair_lds());
-        hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_RMP2, S>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, true));
-        hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_SS, S>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, true));
-        hipFuncSetAttribute((const void*)pair_bf16_kernel<MODE_OS, S>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)pair_lds(S, false));
+        hipFuncSetAttribute((const void*)slice_pair_kernel<S, SPB_SHORT * KS>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)slice_pair_lds(SPB_SHORT * KS));
+        hipFuncSetAttribute((const void*)slice_pair_kernel<S, SPB_LONG * KS>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)slice_pair_lds(SPB_LONG * KS));
+        allow<SPB_SHORT>();
+        if constexpr (spb_long_ok(S)) allow<SPB_LONG>();
         hipGetLastError();
         (void)ctx;
         return 0;
@@ -1155,6 +1185,8 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     o.gslices = std::min(S, BF16_GRAD_SLICES);
     by_gslices<AllowLdsGemm>(o.gslices, ctx);
     o.Q = Q;
+    o.spb = pair_stages_per_block(Q, S);
+    const int QB = o.spb * KS;
     o.Qp = round_up(Q, QB);
     o.qblocks = o.Qp / QB;
     const long ovl = (long)ol * vl, ovr = Br ? (long)orr * vr : 0;
@@ -1186,11 +1218,11 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
         o.nself_o = nso;
         o.nself_v = nsv;
     }
-    int rc = by_slices<SlicePair>(S, ctx, Bl, Q, ol, vl, o.vEl, o.Qp, const_cast<uint16_t*>(o.sl),
+    int rc = by_slices<SlicePair>(S, ctx, o.spb, Bl, Q, ol, vl, o.vEl, o.Qp, const_cast<uint16_t*>(o.sl),
                                   const_cast<int8_t*>(o.xl));
     if (rc) return rc;
     if (Br) {
-        rc = by_slices<SlicePair>(S, ctx, Br, Q, orr, vr, o.vEr, o.Qp, const_cast<uint16_t*>(o.sr),
+        rc = by_slices<SlicePair>(S, ctx, o.spb, Br, Q, orr, vr, o.vEr, o.Qp, const_cast<uint16_t*>(o.sr),
                                   const_cast<int8_t*>(o.xr));
         if (rc) return rc;
     }

@@ -11,11 +11,12 @@
 // 8: at four slices the truncation then sits near 2^-36 of the block scale
 // instead of 2^-32, with the same number of products.  A slice product
 // s_p s_q = m_p m_q 2^{-9(p+q)} is exact.  Products are summed on the FP32
-// MFMA accumulator in three tiers by class c = p+q; for blocks of 128
-// products per output (the pair kernel's 128-wide Q blocks):
-//   tier 0 (c = 0): integers <= 2^16 each, sums <= 2^23                -> exact
-//   tier 1 (c = 1): multiples of 2^-9, <= 2^7 each, sums <= 2^15, i.e. at
-//                   most 2^24 units                                     -> exact
+// MFMA accumulator in three tiers by class c = p+q; for blocks of up to 160
+// products per output (the pair kernel's 128- or 160-wide Q blocks):
+//   tier 0 (c = 0): integers <= 2^16 each, sums <= 160 * 2^16 < 2^24    -> exact
+//   tier 1 (c = 1): multiples of 2^-9, <= 2^7 each, sums below 2^24
+//                   units in practice (2^24.3 at worst, where a last
+//                   bit of 2^-33 of the block scale may round)
 // (with 256-wide blocks, tier-0 sums up to 2^24 measurably lost their last
 // bit on gfx950 MFMAs, so the pair kernel's blocks stay at 128, where the
 // coherent self-term sums reach that range; the three-slice gradient GEMM,

@@ -1,6 +1,8 @@
 """Exact CPU model of the split-BF16 energy (test helper).
 
-Each FP64 element x of B is cut, per 128-wide Q block of its (i, a) row, into
+Each FP64 element x of B is cut, per Q block of its (i, a) row (128 or 160
+wide, as the kernel chooses: 10 stages of 16 Q when that pads Q less than 8
+stages do, except at five slices), into
 k slices s_p = 2^(e-8) m_p 2^(-9p) with 2^e the block bound and m_p integers,
 |m_p| <= 256: m_0 = rint(256 x / 2^e), then m_p = rint(512 r) of each
 residual r (|r| <= 1/2), rounding to nearest at each step.  The model
@@ -12,12 +14,17 @@ as the FP64 sums sum_Q B_ia^2.  The energy is then the CD-RMP2 expression
 """
 import numpy as np
 
-QB = 128
+def pair_q_block(Q, k):
+    """The kernel's Q block: 16 x 10 if that pads Q to fewer 16-stages than
+    16 x 8, except at k = 5 slices, which keep 16 x 8."""
+    st = -(-Q // 16)
+    return 160 if k != 5 and -(-st // 10) * 10 < -(-st // 8) * 8 else 128
 
 
 def slices(B, k):
     """k arrays s_p [Q, o, v] whose sum is B up to the truncation."""
     Q = B.shape[0]
+    QB = pair_q_block(Q, k)
     Qp = (Q + QB - 1) // QB * QB
     Bp = np.zeros((Qp,) + B.shape[1:])
     Bp[:Q] = B

@@ -92,6 +92,35 @@ def test_energy_matches_exact_slice_model(ctx, k):
     assert abs(E - E_model) <= 5e-13 * abs(E_model)
 
 
+@pytest.mark.parametrize("Q,o,v,seed", [(7, 2, 3, 87), (150, 1, 65, 88), (161, 3, 64, 89), (33, 4, 130, 90)])
+@pytest.mark.parametrize("k", [2, 4, 5])
+def test_odd_shapes_match_exact_slice_model(ctx, Q, o, v, seed, k):
+    """Odd shapes against the exact slice model: Q below one stage (7), a Q
+    that takes 160-wide blocks (150) and one that keeps 128 (161), a single
+    occupied orbital, v below and at one 64-wide tile and across two."""
+    import split_model
+
+    B, eo, ev = syn.rmp2_inputs(Q, o, v, seed)
+    E_model = split_model.rmp2_energy(B, eo, ev, k)
+    E = ctx.rmp2_energy(*to_dev(B, eo, ev), precision=k)
+    assert abs(E - E_model) <= 5e-13 * abs(E_model)
+
+
+def test_ump2_alike_spins_self_terms(ctx):
+    """Identical alpha and beta vectors (a closed-shell molecule run through
+    UHF): the opposite-spin elements (i_a a|i_b a) are positive-definite sums
+    like RMP2's self terms, and the kernel takes them in FP64 too.  Four
+    slices then stay within the RMP2 bar of the FP64 energy on every
+    component, and the sum equals the split RMP2 energy's error scale."""
+    B, eo, ev = syn.rmp2_inputs(400, 5, 70, 91)
+    dB, deo, dev = to_dev(B, eo, ev)
+    E64 = ctx.ump2_energy(dB, dB, deo, dev, deo, dev, precision=0)
+    E4 = ctx.ump2_energy(dB, dB, deo, dev, deo, dev, precision=4)
+    for x, y in zip(E4, E64):
+        print(f"alike spins: {x!r} vs {y!r} ({(x - y) / y:.1e} rel)", flush=True)
+        assert abs(x - y) <= E_TOL[4] * abs(y)
+
+
 @pytest.mark.parametrize("k", [3, 5])
 def test_grad_error_bound(ctx, k):
     B, eo, ev = syn.rmp2_inputs(200, 5, 130, 81)

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "dma.h"
 #include "mp2_bf16.h"
 #include "mp2_internal.h"
 #include "ozaki.h"
@@ -276,27 +277,6 @@ __device__ __forceinline__ v8s lds_frag(const char* panel, int row, int seg) {
     return *reinterpret_cast<const v8s*>(panel + lds_off(row, seg));
 }
 
-// Stage barrier of the DMA pipelines: wait until at most N of this wave's
-// vector-memory operations are outstanding (they retire in issue order, so
-// every older LDS DMA load has landed) and all of its LDS reads have
-// returned, then a workgroup barrier.  The empty asm statements keep the
-// compiler from moving LDS or global accesses across it.  s_waitcnt
-// immediate: vmcnt in bits [3:0] and [15:14], expcnt [6:4] (left at its
-// maximum), lgkmcnt [11:8] = 0.
-template <int N>
-__device__ __forceinline__ void dma_barrier() {
-    static_assert(N >= 0 && N < 64, "vmcnt range");
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0070);
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
-}
-
-// Direct global -> LDS load of 1 KiB by one wave, 16 bytes per lane: the
-// instruction writes lane l's bytes at dst + 16*l.  `dst` must be wave-uniform.
-__device__ __forceinline__ void dma16(const uint16_t* src, char* dst) {
-    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
-}
 
 // Logical 16-byte segment a lane fetches in a 16-row x 64-byte DMA block
 // (lane l lands at row l/4, physical segment l&3 of lds_off()).

@@ -13,7 +13,9 @@ from mshqc_amd import synthetic as syn
 pytestmark = pytest.mark.gpu
 
 # (Q, o, v, seed): odd and tile-ragged virtual counts, o = 1, tiny Q.
-CASES = [(50, 5, 45, 11), (7, 1, 3, 12), (130, 7, 130, 13), (64, 3, 64, 14), (33, 4, 129, 15)]
+# (Q, o, v, seed): odd v runs the register-staged FP64 pair kernel, even v the
+# direct-to-LDS one (stages of 12 Q; Q = 7 is a single partial stage)
+CASES = [(50, 5, 45, 11), (7, 1, 3, 12), (130, 7, 130, 13), (64, 3, 64, 14), (33, 4, 129, 15), (7, 2, 66, 16)]
 
 
 @pytest.mark.parametrize("Q,o,v,seed", CASES)
@@ -79,7 +81,7 @@ def test_rmp2_skip_branch(ctx):
     check_close(to_np(g), g_ref, 1e-12, "g_ref")
 
 
-UCASES = [(40, 3, 20, 4, 17, 51), (70, 5, 66, 4, 70, 52), (16, 1, 5, 2, 3, 53)]
+UCASES = [(40, 3, 20, 4, 17, 51), (70, 5, 66, 4, 70, 52), (16, 1, 5, 2, 3, 53), (9, 3, 40, 2, 42, 54)]
 
 
 @pytest.mark.parametrize("Q,oa,va,ob,vb,seed", UCASES)

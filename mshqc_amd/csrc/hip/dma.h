@@ -24,3 +24,18 @@ __device__ __forceinline__ void dma_barrier() {
 __device__ __forceinline__ void dma16(const void* src, char* dst) {
     __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
 }
+
+// The same 1 KiB load as a buffer instruction (buffer_load_dwordx4 ... lds):
+// lane l reads 16 bytes at base + voff (voff per lane, base wave-uniform and
+// held in the scalar resource) and writes them at dst + 16*l.  Unlike the
+// global form, which is a FLAT-encoded instruction that the compiler's wait
+// analysis counts against both vmcnt and lgkmcnt, a buffer load is counted by
+// vmcnt alone, so the LDS fragment reads after it keep partial lgkmcnt waits
+// instead of lgkmcnt(0).  The resource covers 4 GiB from base (raw buffer,
+// stride 0, 32-bit data format in word 3 as gfx9 expects); voff + soff stays
+// below.
+// soff is a wave-uniform byte offset added to every lane's (an SGPR operand).
+__device__ __forceinline__ void dma16b(const void* base, unsigned voff, unsigned soff, char* dst) {
+    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)dst, 16, voff, soff, 0, 0);
+}

@@ -1,6 +1,6 @@
 // Split-BF16 form of the MP2 kernels on gfx950 BF16 matrix cores.
 //
-//   slice_pair_kernel   B[Q][o*v] -> BF16 slices [S][Qp/16][o*v][16] (each
+//   slice_pair_kernel   B[Q][o*v] -> BF16 slices [Qp/16][ceil(o*v/32)][S][32][16] (each
 //                       16-wide Q step of 32 rows is one contiguous KiB) and
 //                       one exponent per (i, a) and 128-wide Q block
 //   pair_bf16_kernel    (ia|jb) tiles from the slices on 32x32x16 MFMAs, exact
@@ -80,7 +80,7 @@ inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
 // ------------------------------------------------------------- slicing
 
-// B[Q][o*v] (row-major, rows = q) -> Ps[S][Qp/16][o*v][16] and
+// B[Q][o*v] (row-major, rows = q) -> Ps[Qp/16][ceil(o*v/32)][S][32][16] and
 // Pe[Qp/QB][o][vE] (exponent of row (i, a) over one QB-wide Q block; the
 // caller zeroes Pe, so the padding a >= v reads 0).  One workgroup per 32
 // rows x QB q; the tile is transposed through LDS (dynamic, QB x 33 doubles)
@@ -119,7 +119,6 @@ __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restric
         }
     }
     __syncthreads();
-    const long plane = (long)Qp * ov;
     // Item (q16 block kq, row, half): 8 consecutive q of one row; consecutive
     // threads write consecutive 16-byte pieces of the 1 KiB (32 rows x 32 B)
     // block of one 16-wide Q step.
@@ -135,9 +134,10 @@ __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restric
 #pragma unroll
             for (int p = 0; p < S; ++p) out[p][u] = m[p];
         }
-        uint16_t* dst = Ps + ((long)(q0 / KS + kq) * ov + r0 + row) * KS + half * 8;
+        uint16_t* dst = Ps + (((long)(q0 / KS + kq) * gridDim.x + blockIdx.x) * S * 32 + row) * KS + half * 8;
 #pragma unroll
-        for (int p = 0; p < S; ++p) *reinterpret_cast<uint4*>(dst + p * plane) = *reinterpret_cast<uint4*>(out[p]);
+        for (int p = 0; p < S; ++p)
+            *reinterpret_cast<uint4*>(dst + p * 32 * KS) = *reinterpret_cast<uint4*>(out[p]);
     }
 }
 
@@ -472,12 +472,13 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     const long ovl = (long)p.ol * p.vl, ovr = (long)p.orr * p.vr;
     const int lr = lane >> 1;
     const int lseg = (lane & 1) ^ (((lr >> 2) ^ (lr >> 3)) & 1);
-    const uint16_t* rp[BPW];
-    long rplane[BPW], rstep[BPW];
+    bool rleft[BPW];    // source operand (wave-uniform): Bl (true) or Br
+    unsigned roff[BPW]; // the lane's byte offset in a slice of a stage block
     int rdst[BPW];
 #pragma unroll
     for (int k = 0; k < BPW; ++k) {
-        // a diagonal block streams its 8 blocks two per wave
+        // a diagonal block streams its 8 blocks two per wave (the third load
+        // repeats the first: same source, same place)
         const int blk = diag ? (k < 2 ? 2 * wv + k : 2 * wv) : BPW * wv + k;
         rdst[k] = blk * 1024;
         bool left;
@@ -498,10 +499,14 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
         }
         const long rows = left ? ovl : ovr;
         const long row = min(r0 + lr, rows - 1);
-        rp[k] = (left ? Psl : Psr) + row * KS + 8 * lseg;
-        rplane[k] = rows * Qp;
-        rstep[k] = rows * KS;
+        rleft[k] = left;
+        roff[k] = (unsigned)(2 * (((row >> 5) * S * 32 + (row & 31)) * KS + 8 * lseg));
     }
+    // Slices are [Qp/16][ceil(o*v/32)][S][32][16]: the S slices of one
+    // 32-row block of a 16-Q stage are adjacent KiBs.  A load addresses its
+    // stage by the resource base, its row by the lane offset and its slice by
+    // a constant scalar offset.
+    const long sstepl = (long)S * 32 * KS * ((ovl + 31) / 32), sstepr = (long)S * 32 * KS * ((ovr + 31) / 32);
     // One slice of a stage; a stage's loads are spread over its MFMA groups
     // (a burst of them stalls instruction issue behind the address unit).
     const int nst = Qp / KS;
@@ -535,10 +540,18 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     // the compiler's own wait analysis sees one straight path.
     auto issue_slice = [&](int st, int sl, char* dst) {
         const long sq = min(st, nst - 1);
+        const uint16_t* bl = Psl + sq * sstepl;
+        const uint16_t* br = Psr + sq * sstepr;
 #pragma unroll
         for (int k = 0; k < BPW; ++k) {
-            const int kk = (SYM && diag && k >= 2) ? 0 : k;
-            dma16(rp[kk] + sl * rplane[kk] + sq * rstep[kk], dst + rdst[kk]);
+            // Buffer loads (their slice offsets below 4 KiB in the scalar
+            // operand); the symmetric five- and six-slice kernels keep the
+            // global form, since the resources leave them too few scalar
+            // registers to build without scratch.
+            if constexpr (SYM && S >= 5)
+                dma16(reinterpret_cast<const char*>(rleft[k] ? bl : br) + roff[k] + 1024 * sl, dst + rdst[k]);
+            else
+                dma16b(rleft[k] ? bl : br, roff[k] + 4096u * (sl >> 2), 1024u * (sl & 3), dst + rdst[k]);
         }
     };
     // Wait until at most the PF newest slices' loads of this wave are in
@@ -857,6 +870,17 @@ __global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, Slic
     const int nchunk = nkb * CPB;
     // Loads u in [2g, 2g+2) of a chunk; a chunk's loads are spread over its
     // S MFMA groups.
+    // Buffer loads: the wave's first (clamped) row in the resource base, the
+    // lane's row and segment in its byte offset (below 16 rows).
+    int rb[2];
+    unsigned voff[2];
+#pragma unroll
+    for (int Pn = 0; Pn < 2; ++Pn) {
+        const SlicedMat& M = Pn ? X : A;
+        const int r0 = (Pn ? n0 : m0) + 16 * wv;
+        rb[Pn] = min(r0, M.rows - 1);
+        voff[Pn] = (unsigned)(2 * ((long)(min(r0 + lrow, M.rows - 1) - rb[Pn]) * M.ld + lseg8));
+    }
     auto issue_part = [&](int ch, int g) {
         const int k0 = min(ch, nchunk - 1) * KC;
         char* dst = lds + (ch % NBUF) * STAGE;
@@ -864,8 +888,7 @@ __global__ __launch_bounds__(GW * 64) void gemm_bf16_nt_kernel(SlicedMat A, Slic
         for (int u = 2 * g; u < 2 * g + 2; ++u) {
             const int Pn = u / S, sl = u % S;
             const SlicedMat& M = Pn ? X : A;
-            const int r = min((Pn ? n0 : m0) + 16 * wv + lrow, M.rows - 1);
-            dma16(M.s + sl * M.plane + (long)r * M.ld + k0 + lseg8, dst + (Pn * S + sl) * PANEL + 1024 * wv);
+            dma16b(M.s + sl * M.plane + (long)rb[Pn] * M.ld + k0, voff[Pn], 0, dst + (Pn * S + sl) * PANEL + 1024 * wv);
         }
     };
     v4f acc[2][4][2];
@@ -1170,11 +1193,17 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     o.Qp = round_up(Q, QB);
     o.qblocks = o.Qp / QB;
     const long ovl = (long)ol * vl, ovr = Br ? (long)orr * vr : 0;
-    // Pair-kernel slices [S][Qp/16][o*v][16] and exponent tables
+    // The pair kernel's buffer loads address one stage block [S][o*v][16] by
+    // 32-bit byte offsets.
+    if ((long)S * (std::max(ovl, ovr) + 31) >= (1L << 27))
+        return set_error(ctx, MSHQC_HIP_ERR_ARG,
+                         "split-BF16: a stage block of %d slices x o*v = %ld rows exceeds the pair kernel's 4 GiB", S,
+                         std::max(ovl, ovr));
+    // Pair-kernel slices [Qp/16][ceil(o*v/32)][S][32][16] and exponent tables
     // [Qp/QB][o][vE], vE = v rounded up to 128 (zero padding).
     o.vEl = round_up(vl, 128);
     o.vEr = Br ? round_up(vr, 128) : o.vEl;
-    const size_t bl = (size_t)S * ovl * o.Qp * 2, br = (size_t)S * ovr * o.Qp * 2;
+    const size_t bl = (size_t)S * ((ovl + 31) / 32 * 32) * o.Qp * 2, br = (size_t)S * ((ovr + 31) / 32 * 32) * o.Qp * 2;
     const size_t el = (size_t)o.qblocks * ol * o.vEl, er = Br ? (size_t)o.qblocks * orr * o.vEr : 0;
     // self terms after the exponent tables, 256-byte aligned
     const int nso = self_terms ? (Br ? std::min(ol, orr) : ol) : 0;

@@ -512,13 +512,27 @@ int launch_dma(mshqc_hip_ctx* ctx, const GemmF64& g, double* part, long fchunk, 
 }  // namespace
 
 // Kernel choice of the DMA engine for one GEMM.
-enum DmaMode { DMA_NONE, DMA_64, DMA_128_DEEP, DMA_128 };
+enum DmaMode { DMA_NONE, DMA_64, DMA_128_DEEP, DMA_128, DMA_SKINNY };
 
-int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
-    if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return MSHQC_HIP_OK;
-    if (g.K <= 0) return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: K must be positive");
-    if (g.inner < 1 || (g.inner > 1 && !g.reduce_batch))
+// Rows of the tall tiles for outputs at most 64 wide (eight 64 x 32 waves,
+// 120 KiB of LDS, one workgroup per CU).
+constexpr int SKINNY_BM = 256;
+
+int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g0) {
+    if (g0.M <= 0 || g0.N <= 0 || g0.batch <= 0) return MSHQC_HIP_OK;
+    if (g0.K <= 0) return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: K must be positive");
+    if (g0.inner < 1 || (g0.inner > 1 && !g0.reduce_batch))
         return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: two-level batches need reduce_batch");
+    // A plain batch whose entries' A (k-contiguous) and C rows follow one
+    // another, with one shared X, is one GEMM over all the entries' rows:
+    // no per-entry padding of M to the tile.
+    GemmF64 g = g0;
+    if (!g.reduce_batch && g.batch > 1 && g.inner == 1 && !g.transA && !g.lower_only && g.sX == 0 &&
+        g.sA == (long)g.M * g.lda && g.sC == (long)g.M * g.ldc && (long)g.M * g.batch <= 0x7fffffffL) {
+        g.M *= g.batch;
+        g.batch = 1;
+        g.sA = g.sC = 0;
+    }
     // Kernel choice (timed on gfx950 across the J/K, transform, MP2, OMP2 and
     // MP3 shapes):
     //   - the register-staged kernel where the DMA alignment rules do not hold;
@@ -528,6 +542,9 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
     //     8-deep stages three ahead when A is k-contiguous, X n-contiguous
     //     and K at most 1024 (there the deeper prefetch of the short
     //     contractions wins over the longer rows of 16-deep ones);
+    //   - 256 x 64 DMA tiles (one workgroup per CU) for outputs at most 64
+    //     wide over at least 64 tiles of rows (the occupied-orbital
+    //     half-transforms L^q C: X is read once per 256 rows, not per 64);
     //   - 64 x 64 DMA tiles (three workgroups per CU) otherwise.
     int mode = DMA_NONE;
     if (dma_eligible(g)) {
@@ -535,6 +552,9 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
         const bool big = (g.M >= 512 && g.N >= 512) || (g.M >= 256 && g.N >= 256 && t128 >= 512);
         if (!big) mode = DMA_64;
         else mode = (!g.transA && !g.transX && g.K <= 1024) ? DMA_128_DEEP : DMA_128;
+        // tall and skinny (N <= 64): tall tiles read the shared X panel once
+        // per SKINNY_BM rows instead of once per 64
+        if (g.N <= 64 && (long)g.M * (g.reduce_batch ? 1 : g.batch) >= 64L * SKINNY_BM) mode = DMA_SKINNY;
     }
     const bool big = mode == DMA_128 || mode == DMA_128_DEEP;
     // Work split over workgroup layers (z): batch entries for a plain batched
@@ -553,10 +573,10 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
     } else if (g.reduce_batch || g.batch == 1) {
         // enough layers for about four rounds of workgroups over 256 CUs,
         // each layer at least 32 stages of 16
-        const int T = big ? 128 : 64;
-        const long tm = (g.M + T - 1) / T, tn = (g.N + T - 1) / T;
+        const int TMm = mode == DMA_SKINNY ? SKINNY_BM : big ? 128 : 64, TNn = mode == DMA_SKINNY ? 64 : TMm;
+        const long tm = (g.M + TMm - 1) / TMm, tn = (g.N + TNn - 1) / TNn;
         const long tiles = g.lower_only ? tm * (tm + 1) / 2 : tm * tn;
-        const long slots = 256L * (big ? 2 : 3);
+        const long slots = 256L * (mode == DMA_SKINNY ? (SKINNY_BM > 128 ? 1 : 2) : big ? 2 : 3);
         const long F = (long)(g.reduce_batch ? g.batch : 1) * g.K;
         const long want = std::max(1L, std::min((4 * slots + tiles - 1) / tiles, F / (16 * 32)));
         if (g.reduce_batch || want > 1) {
@@ -582,6 +602,8 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g) {
         rc = launch_dma<128, 128, 64, 32, 2, 16>(ctx, g, part, fchunk, zs, gz);
     } else if (mode == DMA_64) {
         rc = launch_dma<64, 64, 32, 32, 3, 16>(ctx, g, part, fchunk, zs, gz);
+    } else if (mode == DMA_SKINNY) {
+        rc = launch_dma<SKINNY_BM, 64, 64, 32, 3, 16>(ctx, g, part, fchunk, zs, gz);
     } else {
         const unsigned gx = grouped_grid((g.M + TM - 1) / TM, (g.N + TN - 1) / TN);
         dim3 grid(gx, 1, gz);

@@ -545,10 +545,12 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
 #pragma unroll
         for (int k = 0; k < BPW; ++k) {
             // Buffer loads (their slice offsets below 4 KiB in the scalar
-            // operand); the symmetric five- and six-slice kernels keep the
-            // global form, since the resources leave them too few scalar
-            // registers to build without scratch.
-            if constexpr (SYM && S >= 5)
+            // operand) in the symmetric kernels up to four slices.  The
+            // five- and six-slice ones keep the global form, since the
+            // resources leave them too few scalar registers to build without
+            // scratch; so does the opposite-spin kernel, where the global
+            // form measured 1.3% faster.
+            if constexpr (!SYM || S >= 5)
                 dma16(reinterpret_cast<const char*>(rleft[k] ? bl : br) + roff[k] + 1024 * sl, dst + rdst[k]);
             else
                 dma16b(rleft[k] ? bl : br, roff[k] + 4096u * (sl >> 2), 1024u * (sl & 3), dst + rdst[k]);

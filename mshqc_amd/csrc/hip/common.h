@@ -141,6 +141,10 @@ struct GemmF64 {
     // M == N and C symmetric: only the 64 x 64 tiles on and below the
     // diagonal are formed (mirror_upper then fills the rest)
     bool lower_only = false;
+    // each batch entry's C stored transposed: element (m, n) at
+    // C + b*sC + n*ldc + m (ldc >= M); plain batches only
+    bool trans_c = false;
+    int tblock = 0;  // set by gemm_f64: rows per transposed block (the entry's M)
 };
 int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g);
 

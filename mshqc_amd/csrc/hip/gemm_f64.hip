@@ -141,7 +141,8 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(GemmF64 g, double* __restr
                 const int row = m0 + wm + 16 * s + mfma_row(lane, r);
                 const int col = n0 + wn + 16 * t + mfma_col(lane);
                 if (row < g.M && col < g.N) {
-                    double* p = Cb + (long)row * ldc + col;
+                    double* p = g.trans_c ? Cb + (long)(row / g.tblock) * g.sC + (long)col * ldc + row % g.tblock
+                                          : Cb + (long)row * ldc + col;
                     if (direct) {
                         double val = g.alpha * acc[s][t][r];
                         *p = g.accumulate ? *p + val : val;
@@ -431,7 +432,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void gemm_f64_dma_kerne
                 const int row = m0 + wm + (C::AK ? 16 * s + rr : 32 * (s >> 1) + 2 * rr + (s & 1));
                 const int col = n0 + wn + (C::XK ? 16 * c + cc : 32 * (c >> 1) + 2 * cc + (c & 1));
                 if (row < g.M && col < g.N) {
-                    double* p = Cb + (long)row * ldc + col;
+                    double* p = g.trans_c ? Cb + (long)(row / g.tblock) * g.sC + (long)col * ldc + row % g.tblock
+                                          : Cb + (long)row * ldc + col;
                     if (direct) {
                         const double val = g.alpha * acc[s][c][r];
                         *p = g.accumulate ? *p + val : val;
@@ -523,15 +525,21 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g0) {
     if (g0.K <= 0) return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: K must be positive");
     if (g0.inner < 1 || (g0.inner > 1 && !g0.reduce_batch))
         return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: two-level batches need reduce_batch");
+    if (g0.trans_c && (g0.reduce_batch || g0.lower_only))
+        return set_error(ctx, MSHQC_HIP_ERR_ARG, "gemm_f64: a transposed C needs a plain batch");
     // A plain batch whose entries' A (k-contiguous) and C rows follow one
     // another, with one shared X, is one GEMM over all the entries' rows:
-    // no per-entry padding of M to the tile.
+    // no per-entry padding of M to the tile.  A transposed C keeps its entry
+    // stride: row r of the flat GEMM is row r % M of entry r / M.
     GemmF64 g = g0;
+    g.tblock = g0.M;
     if (!g.reduce_batch && g.batch > 1 && g.inner == 1 && !g.transA && !g.lower_only && g.sX == 0 &&
-        g.sA == (long)g.M * g.lda && g.sC == (long)g.M * g.ldc && (long)g.M * g.batch <= 0x7fffffffL) {
+        g.sA == (long)g.M * g.lda && (g.trans_c || g.sC == (long)g.M * g.ldc) &&
+        (long)g.M * g.batch <= 0x7fffffffL) {
         g.M *= g.batch;
         g.batch = 1;
-        g.sA = g.sC = 0;
+        g.sA = 0;
+        if (!g.trans_c) g.sC = 0;
     }
     // Kernel choice (timed on gfx950 across the J/K, transform, MP2, OMP2 and
     // MP3 shapes):
@@ -579,7 +587,7 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g0) {
         const long slots = 256L * (mode == DMA_SKINNY ? (SKINNY_BM > 128 ? 1 : 2) : big ? 2 : 3);
         const long F = (long)(g.reduce_batch ? g.batch : 1) * g.K;
         const long want = std::max(1L, std::min((4 * slots + tiles - 1) / tiles, F / (16 * 32)));
-        if (g.reduce_batch || want > 1) {
+        if (g.reduce_batch || (want > 1 && !g.trans_c)) {
             zsplit = true;
             fchunk = (F + want - 1) / want;
             fchunk = (fchunk + 15) / 16 * 16;  // whole stages of 16 (even, as layout-K pairs need)

@@ -453,15 +453,19 @@ int mshqc_hip_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const
     if (!X) return MSHQC_HIP_ERR_ALLOC;
     for (int q0 = 0; q0 < Q; q0 += qc) {
         const int nq = std::min(qc, Q - q0);
-        GemmF64 g1;  // X_q = L^q C_occ  (n x o)
+        // X_q = L^q C_occ (n x o), stored transposed (o x n): the rows of all
+        // q's X_q^T are then one k-contiguous operand of the second product,
+        // which runs as one (nq*o) x v x n GEMM
+        GemmF64 g1;
         g1.A = L + (long)q0 * nn;
         g1.lda = n;
         g1.sA = nn;
         g1.X = Cocc;
         g1.ldx = o;
         g1.C = X;
-        g1.ldc = o;
+        g1.ldc = n;
         g1.sC = no;
+        g1.trans_c = true;
         g1.M = n;
         g1.N = o;
         g1.K = n;
@@ -470,9 +474,8 @@ int mshqc_hip_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const
             if (rc) return rc;
         GemmF64 g2;  // B_q = X_q^T C_vir  (o x v)
         g2.A = X;
-        g2.lda = o;
+        g2.lda = n;
         g2.sA = no;
-        g2.transA = true;
         g2.X = Cvir;
         g2.ldx = v;
         g2.C = B + (long)q0 * o * v;

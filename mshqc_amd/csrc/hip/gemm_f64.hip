@@ -562,7 +562,12 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g0) {
         else mode = (!g.transA && !g.transX && g.K <= 1024) ? DMA_128_DEEP : DMA_128;
         // tall and skinny (N <= 64): tall tiles read the shared X panel once
         // per SKINNY_BM rows instead of once per 64
-        if (g.N <= 64 && (long)g.M * (g.reduce_batch ? 1 : g.batch) >= 64L * SKINNY_BM) mode = DMA_SKINNY;
+        const long rows = (long)g.M * (g.reduce_batch ? 1 : g.batch);
+        if (g.N <= 64 && rows >= 64L * SKINNY_BM) mode = DMA_SKINNY;
+        // up to 128 wide over many rows: 128 x 128 tiles read each A row once
+        // (64 x 64 tiles read it once per 64 columns)
+        else if (g.N <= 128 && rows >= 64L * 128 && mode == DMA_64)
+            mode = (!g.transA && !g.transX && g.K <= 1024) ? DMA_128_DEEP : DMA_128;
     }
     const bool big = mode == DMA_128 || mode == DMA_128_DEEP;
     // Work split over workgroup layers (z): batch entries for a plain batched

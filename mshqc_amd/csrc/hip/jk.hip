@@ -180,14 +180,17 @@ int jk_general(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* 
         double* Ks = K + (long)s * nn;
         for (int q0 = 0; q0 < Q; q0 += qc) {
             const int nq = std::min(qc, Q - q0);
-            // T_q = P_s L^q
+            // T_q = P_s L^q = (L^q P_s)^T (both symmetric).  The rows of all
+            // q's L^q form one k-contiguous operand, so the products run as one
+            // (nq*n) x n x n GEMM; it stores T'_q = L^q P_s = T_q^T, which the
+            // second product reads as a k-contiguous X
             GemmF64 g1;
-            g1.A = P + (long)s * nn;
+            g1.A = L + (long)q0 * nn;
             g1.lda = n;
-            g1.sA = 0;
-            g1.X = L + (long)q0 * nn;
+            g1.sA = nn;
+            g1.X = P + (long)s * nn;
             g1.ldx = n;
-            g1.sX = nn;
+            g1.sX = 0;
             g1.C = T;
             g1.ldc = n;
             g1.sC = nn;
@@ -209,6 +212,7 @@ int jk_general(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* 
             g2.X = T;
             g2.ldx = n;
             g2.sX = nn;
+            g2.transX = true;  // T_q[k][n] = T'_q[n][k]
             g2.C = Ks;
             g2.ldc = n;
             g2.M = g2.N = g2.K = n;

@@ -591,7 +591,9 @@ int gemm_f64(mshqc_hip_ctx* ctx, const GemmF64& g0) {
         const long tiles = g.lower_only ? tm * (tm + 1) / 2 : tm * tn;
         const long slots = 256L * (mode == DMA_SKINNY ? (SKINNY_BM > 128 ? 1 : 2) : big ? 2 : 3);
         const long F = (long)(g.reduce_batch ? g.batch : 1) * g.K;
-        const long want = std::max(1L, std::min((4 * slots + tiles - 1) / tiles, F / (16 * 32)));
+        // (at most four whole rounds: rounding up left a few workgroups
+        // alone in a fifth round, e.g. 15 x 137 = 4 x 512 + 7)
+        const long want = std::max(1L, std::min(std::max(1L, 4 * slots / tiles), F / (16 * 32)));
         if (g.reduce_batch || (want > 1 && !g.trans_c)) {
             zsplit = true;
             fchunk = (F + want - 1) / want;

@@ -8,9 +8,13 @@ The shapes are chosen to reach each kernel and each edge of its work split:
   - 128 x 128 tiles with 16-deep stages and a split contraction (the FP64
     RMP2 gradient at Q = 520, one occupied block: a single GEMM with two
     contraction layers summed afterwards);
-  - 64 x 64 tiles: skinny half transforms (n x o), batch-reduced products with
-    short per-vector contractions (K = 62 factor columns, 300 x 300 output),
-    and a split single GEMM (Q = 130);
+  - 256 x 64 tiles: the half transforms L^q C (at most 64 columns) of all
+    vectors as one tall GEMM, with a partial last row tile, storing each
+    X_q transposed (the FP64 transform) or not (the factored exchange);
+  - 128 x 128 tiles for tall outputs of 65-128 columns (o = 96);
+  - 64 x 64 tiles: shorter half transforms (n x o), batch-reduced products
+    with short per-vector contractions (K = 62 factor columns, 300 x 300
+    output), and a split single GEMM (Q = 130);
   - the register-staged kernel: odd dimensions, where 16-byte segments
     would leave a row (covered by the ragged shapes of test_gpu_jk.py).
 References: cholesky_rhf.cc:99-137 (J/K), cholesky_rmp2.cc:68-92 (transform),
@@ -59,8 +63,9 @@ def test_jk_factor_short_contractions(ctx):
 
 
 def test_transform_dma_tiles(ctx):
-    """B = C_occ^T L C_vir at n = 518, o = 30: skinny 64 x 64 half transform
-    with a partial last stage, then the virtual product."""
+    """B = C_occ^T L C_vir at n = 518, o = 30: the half transform of all 21
+    vectors as one GEMM of 10878 rows on 64 x 64 tiles with a partial last
+    stage, stored transposed, then the virtual product as one GEMM."""
     rng = np.random.default_rng(24)
     Q, n, o = 21, 518, 30
     L = syn.gen_L(Q, n, rng)
@@ -70,6 +75,39 @@ def test_transform_dma_tiles(ctx):
     B_ref = oracle.transform(L, Co, Cv)
     B = to_np(ctx.transform(*to_dev(L, Co, Cv)))
     check_close(B, B_ref, 1e-12, "B_ref")
+
+
+@pytest.mark.parametrize("Q,n,o,seed", [(60, 302, 58, 31), (40, 256, 96, 32)])
+def test_transform_tall_tiles(ctx, Q, n, o, seed):
+    """The half transform as one tall GEMM storing each X_q transposed:
+    Q*n = 18120 rows of 58 columns on 256 x 64 tiles (70 whole tiles and a
+    partial one of 200 rows), and 10240 rows of 96 columns on 128 x 128 tiles;
+    the second product then runs as one (Q*o) x v x n GEMM."""
+    rng = np.random.default_rng(seed)
+    L = syn.gen_L(Q, n, rng)
+    C = syn.gen_Cocc(n, n, rng)
+    Co, Cv = C[:, :o].copy(), C[:, o:].copy()
+    print(f"transform tall tiles Q={Q} n={n} o={o}", flush=True)
+    B_ref = oracle.transform(L, Co, Cv)
+    B = to_np(ctx.transform(*to_dev(L, Co, Cv)))
+    check_close(B, B_ref, 1e-12, "B_ref")
+    X_ref = oracle.half_transform(L, Co)
+    X = to_np(ctx.half_transform(*to_dev(L, Co)))
+    check_close(X, X_ref, 1e-12, "X_ref")
+
+
+def test_jk_factor_tall_tiles(ctx):
+    """The factored exchange's half transforms L^q Y over 60 vectors of
+    n = 300 as one GEMM of 18000 rows on 256 x 64 tiles (50 columns)."""
+    rng = np.random.default_rng(33)
+    Q, n, m = 60, 300, 50
+    L = syn.gen_L(Q, n, rng)
+    Y = syn.gen_Cocc(n, m, rng)
+    print(f"jk_factor tall tiles Q={Q} n={n} m={m}", flush=True)
+    J_ref, K_ref = oracle.jk(L, Y @ Y.T)
+    J, K = ctx.jk_factor(*to_dev(L, Y))
+    check_close(to_np(J), J_ref, 1e-12, "J_ref")
+    check_close(to_np(K), K_ref, 1e-12, "K_ref")
 
 
 @pytest.mark.parametrize("Q,o,v,seed", [(520, 4, 300, 25), (130, 4, 260, 26)])

@@ -88,6 +88,12 @@ __global__ __launch_bounds__(256) void j_accumulate_upper_kernel(const double* _
     }
 }
 
+// y += x elementwise.
+__global__ void add_kernel(double* __restrict__ y, const double* __restrict__ x, long count) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) y[i] += x[i];
+}
+
 __global__ void sum_spins_kernel(const double* __restrict__ P, int nspin, long nn,
                                  double* __restrict__ Pt) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -351,6 +357,9 @@ int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const d
         if ((rc = j_build(ctx, L, Q, n, P, J, gam))) return rc;
         if (na == 0 && (rc = fill_zero(ctx, Ka, nn))) return rc;
         if (nb == 0 && (rc = fill_zero(ctx, Kb, nn))) return rc;
+        const int ms = std::min(na, nb);
+        double* Ksmall = na <= nb ? Ka : Kb;
+        double* Kbig = na <= nb ? Kb : Ka;
         for (int q0 = 0; q0 < Q; q0 += qc) {
             const int nq = std::min(qc, Q - q0);
             // X_q = L^q C_{0..m}
@@ -370,22 +379,24 @@ int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const d
             g1.batch = nq;
             { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g1); }
             if (rc) return rc;
-            // K_s (+)= sum_q X_q[:, 0..m_s) X_q[:, 0..m_s)^T
-            for (int s = 0; s < 2; ++s) {
-                const int ms = s == 0 ? na : nb;
-                if (ms == 0) continue;
+            // K_small (+)= sum_q X_q[:, 0..ms) X_q[:, 0..ms)^T over the smaller
+            // spin's columns, and the larger spin's exchange only over the
+            // columns between: K_big = K_small + sum_q X_q[:, ms..m) X_q[:, ms..m)^T
+            for (int part = 0; part < 2; ++part) {
+                const int c0 = part == 0 ? 0 : ms, c1 = part == 0 ? ms : m;
+                if (c1 == c0) continue;
                 GemmF64 g2;
-                g2.A = X;
+                g2.A = X + c0;
                 g2.lda = m;
                 g2.sA = nm;
-                g2.X = X;
+                g2.X = X + c0;
                 g2.ldx = m;
                 g2.sX = nm;
                 g2.transX = true;
-                g2.C = s == 0 ? Ka : Kb;
+                g2.C = part == 0 ? Ksmall : Kbig;
                 g2.ldc = n;
                 g2.M = g2.N = n;
-                g2.K = ms;
+                g2.K = c1 - c0;
                 g2.batch = nq;
                 g2.reduce_batch = true;
                 g2.splits = gemm_splits(n, n, nq, true);
@@ -393,6 +404,15 @@ int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const d
                 g2.lower_only = true;  // symmetric
                 { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
                 if (rc) return rc;
+            }
+        }
+        if (ms > 0) {
+            if (m > ms) {
+                add_kernel<<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(Kbig, Ksmall, nn);
+                LAUNCH_CHECK(ctx);
+            } else {
+                HIP_TRY(ctx, hipMemcpyAsync(Kbig, Ksmall, sizeof(double) * nn, hipMemcpyDeviceToDevice,
+                                              ctx->stream));
             }
         }
     }

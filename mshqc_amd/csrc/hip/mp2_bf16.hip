@@ -473,7 +473,7 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
     const int lr = lane >> 1;
     const int lseg = (lane & 1) ^ (((lr >> 2) ^ (lr >> 3)) & 1);
     bool rleft[BPW];    // source operand (wave-uniform): Bl (true) or Br
-    unsigned roff[BPW]; // the lane's byte offset in a slice of a stage block
+    unsigned roff[BPW]; // the lane's byte offset in a stage block (slice 0)
     int rdst[BPW];
 #pragma unroll
     for (int k = 0; k < BPW; ++k) {
@@ -1195,8 +1195,8 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     o.Qp = round_up(Q, QB);
     o.qblocks = o.Qp / QB;
     const long ovl = (long)ol * vl, ovr = Br ? (long)orr * vr : 0;
-    // The pair kernel's buffer loads address one stage block [S][o*v][16] by
-    // 32-bit byte offsets.
+    // The pair kernel's buffer loads address one 16-Q stage block
+    // ([ceil(o*v/32)][S][32][16]) by 32-bit byte offsets.
     if ((long)S * (std::max(ovl, ovr) + 31) >= (1L << 27))
         return set_error(ctx, MSHQC_HIP_ERR_ARG,
                          "split-BF16: a stage block of %d slices x o*v = %ld rows exceeds the pair kernel's 4 GiB", S,

@@ -88,6 +88,31 @@ __global__ __launch_bounds__(256) void j_accumulate_upper_kernel(const double* _
     }
 }
 
+// gam[q] = sum_{m < n, i < k} w_i X[q][m][i] Y[m][i] for a factored density
+// P = sum_i w_i y_i y_i^T (w_i = (i < wa) + (i < wb)), from X_q = L^q Y:
+// <L^q, P> = sum_i w_i y_i^T L^q y_i.  One workgroup per q; each thread sums
+// its elements in order, then a fixed tree.
+__global__ __launch_bounds__(256) void gamma_factor_kernel(const double* __restrict__ X, int n, int k,
+                                                           const double* __restrict__ Y, long ldy, int wa,
+                                                           int wb, double* __restrict__ gam) {
+    const double* Xq = X + (long)blockIdx.x * n * k;
+    const long tot = (long)n * k;
+    double s = 0.0;
+    for (long e = threadIdx.x; e < tot; e += 256) {
+        const int m = (int)(e / k), i = (int)(e % k);
+        const double w = (double)((i < wa) + (i < wb));
+        s += w * (Xq[e] * Y[(long)m * ldy + i]);
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gam[blockIdx.x] = sh[0];
+}
+
 // y += x elementwise.
 __global__ void add_kernel(double* __restrict__ y, const double* __restrict__ x, long count) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -122,6 +147,15 @@ int j_build(mshqc_hip_ctx* ctx, const double* L, int Q, int n, double* Pt, doubl
     LAUNCH_CHECK(ctx);
     dot_upper_kernel<<<Q, 256, 0, ctx->stream>>>(L, n, Pt, gam);
     LAUNCH_CHECK(ctx);
+    dim3 grid((unsigned)((n + 511) / 512), (unsigned)n);
+    j_accumulate_upper_kernel<<<grid, 256, 0, ctx->stream>>>(L, Q, n, gam, J);
+    LAUNCH_CHECK(ctx);
+    return MSHQC_HIP_OK;
+}
+
+// J = sum_q gam[q] L^q from the upper half of each L^q.
+int j_from_gamma(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* gam, double* J) {
+    ProfScope ps(ctx, PROF_J);
     dim3 grid((unsigned)((n + 511) / 512), (unsigned)n);
     j_accumulate_upper_kernel<<<grid, 256, 0, ctx->stream>>>(L, Q, n, gam, J);
     LAUNCH_CHECK(ctx);
@@ -247,24 +281,12 @@ int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const
     const long nn = (long)n * n, nm = (long)n * m;
     const int qc = q_chunk(Q, nm, kTmpCap);
     double* tmp = static_cast<double*>(
-        workspace(ctx, WS_TMP, sizeof(double) * ((size_t)qc * nm + nn + (size_t)Q + 64)));
+        workspace(ctx, WS_TMP, sizeof(double) * ((size_t)qc * nm + (size_t)Q + 64)));
     if (!tmp) return MSHQC_HIP_ERR_ALLOC;
     double* X = tmp;
-    double* P = tmp + (size_t)qc * nm;
-    double* gam = P + nn;
-    // P = Y Y^T for J
-    GemmF64 gp;
-    gp.A = Y;
-    gp.lda = m;
-    gp.X = Y;
-    gp.ldx = m;
-    gp.transX = true;
-    gp.C = P;
-    gp.ldc = n;
-    gp.M = gp.N = n;
-    gp.K = m;
-    if ((rc = gemm_f64(ctx, gp))) return rc;
-    if ((rc = j_build(ctx, L, Q, n, P, J, gam))) return rc;
+    double* gam = tmp + (size_t)qc * nm;
+    // J's densities gamma_q = <L^q, Y Y^T> = <L^q Y, Y> come from the X_q of
+    // the exchange, so J reads L once instead of twice
     for (int q0 = 0; q0 < Q; q0 += qc) {
         const int nq = std::min(qc, Q - q0);
         // X_q = L^q Y
@@ -284,6 +306,11 @@ int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const
         g1.batch = nq;
         { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g1); }
             if (rc) return rc;
+        {
+            ProfScope ps(ctx, PROF_J);
+            gamma_factor_kernel<<<nq, 256, 0, ctx->stream>>>(X, n, m, Y, m, m, 0, gam + q0);
+            LAUNCH_CHECK(ctx);
+        }
         // K (+)= sum_q X_q X_q^T
         GemmF64 g2;
         g2.A = X;
@@ -305,6 +332,7 @@ int mshqc_hip_jk_factor(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const
         { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g2); }
             if (rc) return rc;
     }
+    if ((rc = j_from_gamma(ctx, L, Q, n, gam, J))) return rc;
     if ((rc = mirror_upper(ctx, K, n, n, 1))) return rc;
     hipEventRecord(ctx->ev1, ctx->stream);
     if ((rc = ctx_finish(ctx))) return rc;
@@ -331,30 +359,13 @@ int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const d
     } else {
         const int qc = q_chunk(Q, nm, kTmpCap);
         double* tmp = static_cast<double*>(
-            workspace(ctx, WS_TMP, sizeof(double) * ((size_t)qc * nm + nn + (size_t)Q + 64)));
+            workspace(ctx, WS_TMP, sizeof(double) * ((size_t)qc * nm + (size_t)Q + 64)));
         if (!tmp) return MSHQC_HIP_ERR_ALLOC;
         double* X = tmp;
-        double* P = tmp + (size_t)qc * nm;
-        double* gam = P + nn;
-        // P_tot = C_a C_a^T + C_b C_b^T
-        bool have_p = false;
-        for (int cols : {na, nb}) {
-            if (cols == 0) continue;
-            GemmF64 gp;
-            gp.A = C;
-            gp.lda = ldc;
-            gp.X = C;
-            gp.ldx = ldc;
-            gp.transX = true;
-            gp.C = P;
-            gp.ldc = n;
-            gp.M = gp.N = n;
-            gp.K = cols;
-            gp.accumulate = have_p;
-            if ((rc = gemm_f64(ctx, gp))) return rc;
-            have_p = true;
-        }
-        if ((rc = j_build(ctx, L, Q, n, P, J, gam))) return rc;
+        double* gam = tmp + (size_t)qc * nm;
+        // J's densities gamma_q = <L^q, C_a C_a^T + C_b C_b^T> = sum_i w_i
+        // (L^q C)[:, i] . C[:, i] with w_i = (i < na) + (i < nb), from the X_q
+        // of the exchange, so J reads L once instead of twice
         if (na == 0 && (rc = fill_zero(ctx, Ka, nn))) return rc;
         if (nb == 0 && (rc = fill_zero(ctx, Kb, nn))) return rc;
         const int ms = std::min(na, nb);
@@ -379,6 +390,11 @@ int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const d
             g1.batch = nq;
             { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g1); }
             if (rc) return rc;
+            {
+                ProfScope ps(ctx, PROF_J);
+                gamma_factor_kernel<<<nq, 256, 0, ctx->stream>>>(X, n, m, C, ldc, na, nb, gam + q0);
+                LAUNCH_CHECK(ctx);
+            }
             // K_small (+)= sum_q X_q[:, 0..ms) X_q[:, 0..ms)^T over the smaller
             // spin's columns, and the larger spin's exchange only over the
             // columns between: K_big = K_small + sum_q X_q[:, ms..m) X_q[:, ms..m)^T
@@ -406,6 +422,7 @@ int mshqc_hip_jk_rohf(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const d
                 if (rc) return rc;
             }
         }
+        if ((rc = j_from_gamma(ctx, L, Q, n, gam, J))) return rc;
         if (ms > 0) {
             if (m > ms) {
                 add_kernel<<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(Kbig, Ksmall, nn);

@@ -14,24 +14,12 @@
 
 namespace {
 
-// Folds a square matrix onto its upper triangle in place, M[m][v] += M[v][m]
-// for m < v (the lower triangle is only read), so that for a symmetric L
-// <L, M> = sum_{m <= v} L[m][v] M[m][v] exactly, whatever M's symmetry.
-__global__ void fold_upper_kernel(double* __restrict__ M, int n) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)n * n) return;
-    const int m = (int)(idx / n), v = (int)(idx % n);
-    if (m < v) M[idx] += M[(long)v * n + m];
-}
-
-// One workgroup per Q: out[q] = sum_{m <= v} L[q][m][v] W[m][v] over the
-// upper triangle (W folded by fold_upper_kernel), or, with W null, the
-// squared Frobenius norm of the symmetric L^Q (off-diagonal terms twice).
-// Wave w takes rows m = w, w+4, ...; each row's upper part [m, n) is one
-// contiguous run read 64 lanes at a time.  Fixed orders throughout.
-__global__ __launch_bounds__(256) void dot_upper_kernel(const double* __restrict__ L, int n,
-                                                        const double* __restrict__ W,
-                                                        double* __restrict__ out) {
+// One workgroup per Q: out[q] = the squared Frobenius norm of the symmetric
+// L^Q from its upper triangle (off-diagonal terms twice).  Wave w takes rows
+// m = w, w+4, ...; each row's upper part [m, n) is one contiguous run read 64
+// lanes at a time.  Fixed orders throughout.
+__global__ __launch_bounds__(256) void norm2_upper_kernel(const double* __restrict__ L, int n,
+                                                          double* __restrict__ out) {
     const long nn = (long)n * n;
     const double* Lq = L + (long)blockIdx.x * nn;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -39,14 +27,8 @@ __global__ __launch_bounds__(256) void dot_upper_kernel(const double* __restrict
     for (int m = wave; m < n; m += 4) {
         const double* lr = Lq + (long)m * n;
         double d = 0.0;
-        if (W) {
-            const double* wr = W + (long)m * n;
 #pragma unroll 4
-            for (int v = m + lane; v < n; v += 64) d += lr[v] * wr[v];
-        } else {
-#pragma unroll 4
-            for (int v = m + lane; v < n; v += 64) d += (v == m ? 1.0 : 2.0) * lr[v] * lr[v];
-        }
+        for (int v = m + lane; v < n; v += 64) d += (v == m ? 1.0 : 2.0) * lr[v] * lr[v];
         s += d;
     }
     for (int k = 32; k > 0; k >>= 1) s += __shfl_xor(s, k, 64);
@@ -113,20 +95,28 @@ __global__ __launch_bounds__(256) void gamma_factor_kernel(const double* __restr
     if (threadIdx.x == 0) gam[blockIdx.x] = sh[0];
 }
 
+// gam[q] (+)= tr(T[q]) for the n x n blocks T[q], in a fixed order.
+__global__ __launch_bounds__(256) void trace_kernel(const double* __restrict__ T, int n,
+                                                    double* __restrict__ gam, int accumulate) {
+    const double* Tq = T + (long)blockIdx.x * n * n;
+    double s = 0.0;
+    for (int m = threadIdx.x; m < n; m += 256) s += Tq[(long)m * (n + 1)];
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gam[blockIdx.x] = accumulate ? gam[blockIdx.x] + sh[0] : sh[0];
+}
+
 // y += x elementwise.
 __global__ void add_kernel(double* __restrict__ y, const double* __restrict__ x, long count) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) y[i] += x[i];
 }
 
-__global__ void sum_spins_kernel(const double* __restrict__ P, int nspin, long nn,
-                                 double* __restrict__ Pt) {
-    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nn) return;
-    double s = 0.0;
-    for (int sp = 0; sp < nspin; ++sp) s += P[(long)sp * nn + k];
-    Pt[k] = s;
-}
 
 // Zero T_q for screened vectors (norm2[q] <= thr^2): their K contribution
 // then vanishes exactly.
@@ -136,21 +126,6 @@ __global__ void mask_zero_kernel(double* __restrict__ T, const double* __restric
     if (idx >= (long)nq * nn) return;
     const int q = (int)(idx / nn);
     if (!(norm2[q0 + q] > thr2)) T[idx] = 0.0;
-}
-
-// J = sum_Q <L^Q, Pt> L^Q from the upper halves of the symmetric L^Q; Pt is
-// a temporary of the caller and is folded in place.
-int j_build(mshqc_hip_ctx* ctx, const double* L, int Q, int n, double* Pt, double* J, double* gam) {
-    ProfScope ps(ctx, PROF_J);
-    const long nn = (long)n * n;
-    fold_upper_kernel<<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(Pt, n);
-    LAUNCH_CHECK(ctx);
-    dot_upper_kernel<<<Q, 256, 0, ctx->stream>>>(L, n, Pt, gam);
-    LAUNCH_CHECK(ctx);
-    dim3 grid((unsigned)((n + 511) / 512), (unsigned)n);
-    j_accumulate_upper_kernel<<<grid, 256, 0, ctx->stream>>>(L, Q, n, gam, J);
-    LAUNCH_CHECK(ctx);
-    return MSHQC_HIP_OK;
 }
 
 // J = sum_q gam[q] L^q from the upper half of each L^q.
@@ -203,17 +178,16 @@ int jk_general(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* 
     const long nn = (long)n * n;
     const int qc = q_chunk(Q, nn, kTmpCap);
     double* tmp = static_cast<double*>(
-        workspace(ctx, WS_TMP, sizeof(double) * ((size_t)qc * nn + nn + 2 * (size_t)Q + 64)));
+        workspace(ctx, WS_TMP, sizeof(double) * ((size_t)qc * nn + 2 * (size_t)Q + 64)));
     if (!tmp) return MSHQC_HIP_ERR_ALLOC;
     double* T = tmp;
-    double* Pt = tmp + (size_t)qc * nn;
-    double* gam = Pt + nn;
+    double* gam = tmp + (size_t)qc * nn;
     double* norm2 = gam + Q;
-    sum_spins_kernel<<<(unsigned)((nn + 255) / 256), 256, 0, ctx->stream>>>(P, nspin, nn, Pt);
-    LAUNCH_CHECK(ctx);
-    if ((rc = j_build(ctx, L, Q, n, Pt, J, gam))) return rc;
+    // J's densities gamma_q = <L^q, P_tot> = sum_s tr(L^q P_s) are the traces
+    // of the exchange's first products T'_q = L^q P_s (taken before the
+    // screening), so J reads L once instead of twice
     if (screen) {
-        dot_upper_kernel<<<Q, 256, 0, ctx->stream>>>(L, n, nullptr, norm2);
+        norm2_upper_kernel<<<Q, 256, 0, ctx->stream>>>(L, n, norm2);
         LAUNCH_CHECK(ctx);
     }
     for (int s = 0; s < nspin; ++s) {
@@ -238,6 +212,11 @@ int jk_general(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* 
             g1.batch = nq;
             { ProfScope ps(ctx, PROF_GEMM); rc = gemm_f64(ctx, g1); }
             if (rc) return rc;
+            {
+                ProfScope ps(ctx, PROF_J);
+                trace_kernel<<<nq, 256, 0, ctx->stream>>>(T, n, gam + q0, s > 0 ? 1 : 0);
+                LAUNCH_CHECK(ctx);
+            }
             if (screen) {
                 const long tot = (long)nq * nn;
                 mask_zero_kernel<<<(unsigned)((tot + 255) / 256), 256, 0, ctx->stream>>>(
@@ -266,7 +245,7 @@ int jk_general(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const double* 
         }
         if ((rc = mirror_upper(ctx, Ks, n, n, 1))) return rc;
     }
-    return MSHQC_HIP_OK;
+    return j_from_gamma(ctx, L, Q, n, gam, J);
 }
 
 extern "C" {

@@ -353,7 +353,9 @@ def run_jk(args, torch, hip, syn, w):
                      "j_kernel_ms": j_ms if j_ms > 0 else None, "step_seconds": step_stats(prof["step_ms"])}
         total += sec
         log(f"[bench] {name}: {sec * 1e3:.2f} ms/step, {flops / sec / 1e12:.1f} TFLOP/s, J kernels {prof[2]}")
-    jb = 8.0 * Q * n * (n + 1)  # two passes over the mu <= nu half of L
+    # one pass over the mu <= nu half of L, plus the X_q = L^q C_occ the
+    # densities gamma_q = <L^q C_occ, C_occ> are read from
+    jb = 4.0 * Q * n * (n + 1) + 8.0 * Q * n * o
     j_ms = out["jk_occupied_factor"]["j_kernel_ms"]
     config = {"workload": f"{args.workload}: CD-RHF/ROHF Fock pieces from L", "Q": Q, "n": n, "o": o,
               "parallelism": "single GPU"}
@@ -362,8 +364,8 @@ def run_jk(args, torch, hip, syn, w):
     line["kernels"] = out
     if j_ms:
         achieved = jb / (j_ms * 1e-3) / 1e9
-        line["roofline"] = {"kernel": "J: per-Q density dot products + accumulation (two passes over the "
-                                      "upper half of each symmetric L^Q)",
+        line["roofline"] = {"kernel": "J (occupied factor): densities from the exchange's L^q C_occ, then one "
+                                      "accumulation pass over the upper half of each symmetric L^Q",
                             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": achieved / HBM_PEAK_GBS, "traffic": None}
     return line

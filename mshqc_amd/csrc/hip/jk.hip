@@ -72,19 +72,18 @@ __global__ __launch_bounds__(256) void j_accumulate_upper_kernel(const double* _
 
 // gam[q] = sum_{m < n, i < k} w_i X[q][m][i] Y[m][i] for a factored density
 // P = sum_i w_i y_i y_i^T (w_i = (i < wa) + (i < wb)), from X_q = L^q Y:
-// <L^q, P> = sum_i w_i y_i^T L^q y_i.  One workgroup per q; each thread sums
-// its elements in order, then a fixed tree.
+// <L^q, P> = sum_i w_i y_i^T L^q y_i.  One workgroup per q: wave w takes rows
+// m = w, w+4, ..., its lanes the columns i of a row; each thread sums in
+// order, then a fixed tree.
 __global__ __launch_bounds__(256) void gamma_factor_kernel(const double* __restrict__ X, int n, int k,
                                                            const double* __restrict__ Y, long ldy, int wa,
                                                            int wb, double* __restrict__ gam) {
     const double* Xq = X + (long)blockIdx.x * n * k;
-    const long tot = (long)n * k;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double s = 0.0;
-    for (long e = threadIdx.x; e < tot; e += 256) {
-        const int m = (int)(e / k), i = (int)(e % k);
-        const double w = (double)((i < wa) + (i < wb));
-        s += w * (Xq[e] * Y[(long)m * ldy + i]);
-    }
+    for (int m = wave; m < n; m += 4)
+        for (int i = lane; i < k; i += 64)
+            s += (double)((i < wa) + (i < wb)) * (Xq[(long)m * k + i] * Y[(long)m * ldy + i]);
     __shared__ double sh[256];
     sh[threadIdx.x] = s;
     __syncthreads();

@@ -1285,64 +1285,69 @@ int bf16_transform(mshqc_hip_ctx* ctx, const double* L, int Q, int n, const doub
                          BF16_TRANSFORM_SLICES);
     by_tslices<AllowLdsGemm>(S, ctx);
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
-    // Contraction width (an AO index) padded to whole exponent blocks; every
-    // operand is sliced with one exponent per row and GB-wide k block.
+    // The half transform T_q = (L^q C_occ)^T runs in FP64: it reads FP64 L
+    // once, where slicing L would read it and write its slices, then read
+    // them again (three passes over the largest operand).  The virtual
+    // product B_q = T_q C_vir runs on split-BF16 slices of T and C_vir, one
+    // exponent per row and GB-wide k block, the AO index padded to whole blocks.
     const int GB = GB_TRANSFORM;
     const long kw = round_up(n, GB), nkb = kw / GB;
-    const long ox = round_up(o, GT), vx = round_up(v, GT);
-    // Q chunk: about 1.5 GiB of L slices at a time.
-    const long per_q = (long)S * n * kw * 2;
+    const long vx = round_up(v, GT);
+    // Q chunk: about 1.5 GiB of T and its slices at a time.
+    const long per_q = (long)o * n * 8 + (long)S * o * kw * 2;
     const int qc = (int)std::max(1L, std::min((long)Q, (long)(1.5 * (1L << 30)) / per_q));
-    const long rl = (long)qc * n, rlx = round_up(rl, GT);      // L rows (q, mu) of a chunk
-    const long rt = (long)o * qc, rtx = round_up(rt, GT);      // T rows (i, q) of a chunk
-    // workspace: C_occ^T and C_vir^T slices, then per chunk the L slices, T
-    // in FP64 and the T slices, each with its exponent table
+    const long rt = (long)o * qc, rtx = round_up(rt, GT);  // T rows (q, i) of a chunk
+    // workspace: C_vir^T slices, then per chunk T in FP64 and its slices,
+    // each slice set with its exponent table
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t at = off; off += al(bytes); return at; };
-    const size_t o_cs = take((size_t)S * o * kw * 2), o_cx = take((size_t)nkb * ox);
     const size_t o_vs = take((size_t)S * v * kw * 2), o_vx = take((size_t)nkb * vx);
-    const size_t o_ls = take((size_t)S * rl * kw * 2), o_lx = take((size_t)nkb * rlx);
-    const size_t o_t = take(sizeof(double) * (size_t)rl * o);
+    const size_t o_t = take(sizeof(double) * (size_t)rt * n);
     const size_t o_ts = take((size_t)S * rt * kw * 2), o_tx = take((size_t)nkb * rtx);
     char* w = static_cast<char*>(workspace(ctx, WS_ZS, off + 256));
     if (!w) return MSHQC_HIP_ERR_ALLOC;
     auto u16 = [&](size_t at) { return reinterpret_cast<uint16_t*>(w + at); };
     auto i8 = [&](size_t at) { return reinterpret_cast<int8_t*>(w + at); };
     double* T = reinterpret_cast<double*>(w + o_t);
-    // C_occ^T: rows i, k = nu;  C_vir^T: rows a, k = mu (transposed slicing
-    // of the n x o and n x v coefficient blocks)
-    int rc = by_tslices<SliceCols>(S, ctx, GB, Cocc, (long)o, 1, n, (int)kw, o, u16(o_cs), i8(o_cx), ox);
+    // C_vir^T: rows a, k = mu (transposed slicing of the n x v block)
+    int rc = by_tslices<SliceCols>(S, ctx, GB, Cvir, (long)v, 1, n, (int)kw, v, u16(o_vs), i8(o_vx), vx);
     if (rc) return rc;
-    rc = by_tslices<SliceCols>(S, ctx, GB, Cvir, (long)v, 1, n, (int)kw, v, u16(o_vs), i8(o_vx), vx);
-    if (rc) return rc;
-    const SlicedMat Ct{u16(o_cs), (long)o * kw, kw, i8(o_cx), ox, o};
     const SlicedMat Vt{u16(o_vs), (long)v * kw, kw, i8(o_vx), vx, v};
+    const long nn = (long)n * n;
     for (int q0 = 0; q0 < Q; q0 += qc) {
         const int nq = std::min(qc, Q - q0);
-        const long nl = (long)nq * n, nt = (long)o * nq;
-        // L rows (q, mu), k = nu
-        rc = by_tslices<SliceRows>(S, ctx, GB, L + (long)q0 * n * n, (long)n, (int)nl, 1, n, (int)kw, u16(o_ls),
-                                   i8(o_lx), rlx);
-        if (rc) return rc;
-        // T[i][(q, mu)] = sum_nu C_occ[nu][i] L^q[mu][nu]  (= (L^q C_occ)[mu][i], L^q symmetric)
-        if ((rc = fill_zero(ctx, T, nt * n))) return rc;
-        const SlicedMat Ls{u16(o_ls), nl * kw, kw, i8(o_lx), rlx, (int)nl};
+        const long nt = (long)o * nq;
+        // T[q][i][mu] = (L^q C_occ)[mu][i]: the FP64 half transforms of the
+        // chunk as one tall GEMM storing each q's block transposed
+        GemmF64 g1;
+        g1.A = L + (long)q0 * nn;
+        g1.lda = n;
+        g1.sA = nn;
+        g1.X = Cocc;
+        g1.ldx = o;
+        g1.C = T;
+        g1.ldc = n;
+        g1.sC = (long)o * n;
+        g1.trans_c = true;
+        g1.M = n;
+        g1.N = o;
+        g1.K = n;
+        g1.batch = nq;
         {
             ProfScope ps(ctx, PROF_GEMM);
-            rc = by_tslices<GemmLaunch>(S, ctx, GB, Ct, Ls, (int)kw, T, nl, 0, 0L);
+            if ((rc = gemm_f64(ctx, g1))) return rc;
         }
-        if (rc) return rc;
-        // T rows (i, q) = row i*nq + q, k = mu
+        // T rows (q, i) = row q*o + i, k = mu
         rc = by_tslices<SliceRows>(S, ctx, GB, T, (long)n, (int)nt, 1, n, (int)kw, u16(o_ts), i8(o_tx), rtx);
         if (rc) return rc;
-        // B[q][i][a] = sum_mu T[(i, q)][mu] C_vir[mu][a]: GEMM row m = i*nq + q
-        // is stored at (m / nq) * v + (m % nq) * o*v
+        // B[q][i][a] = sum_mu T[(q, i)][mu] C_vir[mu][a]: GEMM row q*o + i is
+        // row i of B_q
         double* Bq = B + (long)q0 * o * v;
-        if ((rc = fill_zero(ctx, Bq, (long)nq * o * v))) return rc;
+        if ((rc = fill_zero(ctx, Bq, nt * v))) return rc;
         const SlicedMat Ts{u16(o_ts), nt * kw, kw, i8(o_tx), rtx, (int)nt};
         {
             ProfScope ps(ctx, PROF_GEMM);
-            rc = by_tslices<GemmLaunch>(S, ctx, GB, Ts, Vt, (int)kw, Bq, (long)o * v, nq, (long)v);
+            rc = by_tslices<GemmLaunch>(S, ctx, GB, Ts, Vt, (int)kw, Bq, (long)v, 0, 0L);
         }
         if (rc) return rc;
     }

@@ -52,9 +52,13 @@ def test_transform_split_bf16(ctx, Q, n, o, seed, k):
 
 
 def test_transform_split_bf16_energy(ctx):
-    """The four-slice transform's effect on the CD-RMP2 energy it feeds: the
-    FP64 energy of the split-BF16 B within 1e-12 relative of that of the FP64
-    B, and so far below the 1 uEh bar at this size (|E| ~ 1e2 Eh)."""
+    """The four-slice transform's effect on the CD-RMP2 energy it feeds.  The
+    half transform is FP64; the virtual product cuts each row block of T and
+    C_vir at 2^-35 of its bound, so B carries ~1e-9 relative error (max norm),
+    random in sign.  The energy's error is a random sum over B's errors,
+    measured at 1e-12 to 3e-12 relative at this shape (|E| ~ 4e3 Eh) and
+    9e-14 to 1.5e-13 at o=100/v=1100/Q=4000: the test allows 1e-11, i.e.
+    4e-8 Eh here, far below the 1 uEh bar."""
     L, Co, Cv = _inputs(400, 260, 20, 14)
     eo = np.sort(np.random.default_rng(15).uniform(-1.5, -0.3, 20))
     ev = 0.1 + 3.0 * (np.arange(240) / 240) ** 1.5
@@ -62,7 +66,7 @@ def test_transform_split_bf16_energy(ctx):
     E64 = ctx.rmp2_energy(ctx.transform(dL, dCo, dCv), deo, dev)
     E4 = ctx.rmp2_energy(ctx.transform(dL, dCo, dCv, precision=4), deo, dev)
     print(f"energy from the four-slice transform: {E4!r} vs {E64!r} ({(E4 - E64) / E64:.1e} rel)", flush=True)
-    assert abs(E4 - E64) <= 1e-12 * abs(E64)
+    assert abs(E4 - E64) <= 1e-11 * abs(E64)
 
 
 @pytest.mark.parametrize("Q,n,nspin,screen,seed", [(23, 37, 1, True, 4), (31, 50, 2, False, 5),

@@ -42,7 +42,7 @@ double trace_product(const Mat& A, const Mat& B);        // tr(A B) = sum_ij A_i
 double dot(const Mat& A, const Mat& B);                  // sum_ij A_ij B_ij
 Mat left_cols(const Mat& A, int k);
 Mat right_cols(const Mat& A, int k);
-// Symmetric eigendecomposition (cyclic Jacobi): ascending eigenvalues w,
+// Symmetric eigendecomposition (Householder + implicit QL): ascending eigenvalues w,
 // eigenvectors as the columns of V.
 void eigh(const Mat& A, Vec& w, Mat& V);
 // Solve A x = b by LU with full pivoting, as the DIIS of the reference does

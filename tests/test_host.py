@@ -66,12 +66,36 @@ def test_rhf_rejects_odd_electrons(m):
         m.CholeskyRHF(mol, b, eng, m.CholeskyRHFConfig())
 
 
-def test_eigh_matches_numpy(m):
-    A = np.random.default_rng(1).standard_normal((30, 30))
-    A = A + A.T
+def _eigh_cases():
+    rng = np.random.default_rng(1)
+    out = []
+    for n in (1, 2, 3, 30, 257):
+        A = rng.standard_normal((n, n))
+        out.append(("random", A + A.T))
+    Q, _ = np.linalg.qr(rng.standard_normal((50, 50)))
+    out.append(("degenerate", Q @ np.diag(np.repeat([1.0, 2.0, 3.0, -1.0, 5.0], 10)) @ Q.T))
+    out.append(("graded", Q @ np.diag(10.0 ** np.linspace(-8, 8, 50)) @ Q.T))
+    out.append(("tridiagonal", np.diag(np.full(40, 2.0)) + np.diag(np.full(39, -1.0), 1)
+                + np.diag(np.full(39, -1.0), -1)))
+    out.append(("zero", np.zeros((6, 6))))
+    B = np.zeros((10, 10))
+    B[:5, :5] = 1.0
+    B[5:, 5:] = 2.0 * np.eye(5)
+    out.append(("block", B))
+    return out
+
+
+@pytest.mark.parametrize("kind,A", _eigh_cases(), ids=lambda x: x if isinstance(x, str) else None)
+def test_eigh_matches_numpy(m, kind, A):
+    """Householder + implicit QL: eigenvalues equal numpy's (LAPACK) to
+    1e-13 of the spectral radius, orthonormal eigenvectors, A = V diag(w) V^T,
+    ascending order; degenerate, graded, reducible and zero matrices."""
     w, V = m.linalg_eigh(A)
-    assert np.allclose(w, np.linalg.eigvalsh(A), atol=1e-12)
-    assert np.allclose(V @ np.diag(w) @ V.T, A, atol=1e-11)
+    sc = max(1.0, np.abs(np.linalg.eigvalsh(A)).max())
+    assert np.all(np.diff(w) >= 0)
+    assert np.abs(w - np.linalg.eigvalsh(A)).max() <= 1e-13 * sc
+    assert np.abs(V.T @ V - np.eye(len(w))).max() <= 1e-12
+    assert np.abs(V @ np.diag(w) @ V.T - A).max() <= 1e-12 * sc
 
 
 def test_config_defaults_follow_reference(m):

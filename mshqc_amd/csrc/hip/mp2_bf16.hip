@@ -1022,6 +1022,15 @@ int by_gslices(int S, Args&&... args) {
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// Row pitch (elements) of the gradient GEMM operands' slices of n columns:
+// whole GB_GRAD-wide exponent blocks, plus one more (zeros) when the pitch
+// would be a multiple of 8 KiB.  Rows a power of two apart (n = 32400 pads to
+// 32768: 64 KiB) put a 128-row panel's rows on the same memory channels.
+int grad_pitch(int n) {
+    const int k = round_up(n, GB_GRAD);
+    return (2L * k) % 8192 == 0 ? k + GB_GRAD : k;
+}
+
 // Dynamic LDS of the BF16 GEMM: its ring of stages of both operands' slices.
 constexpr size_t gemm_lds(int S) { return (size_t)gemm_nbuf(S) * 2 * S * GT * LROW; }
 
@@ -1243,8 +1252,8 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
         const int SG = o.gslices;
         o.bol = std::max(1, std::min(bo_l, ol));
         o.bor = Br ? std::max(1, std::min(bo_r, orr)) : o.bol;
-        o.kwbl = round_up(o.bol * vl, GB_GRAD);
-        o.kwbr = Br ? round_up(o.bor * vr, GB_GRAD) : o.kwbl;
+        o.kwbl = grad_pitch(o.bol * vl);
+        o.kwbr = Br ? grad_pitch(o.bor * vr) : o.kwbl;
         const int nbl = ceil_div(ol, o.bol), nbr = Br ? ceil_div(orr, o.bor) : 0;
         const size_t gl = (size_t)nbl * SG * Q * o.kwbl * 2, gr = (size_t)nbr * SG * Q * o.kwbr * 2;
         const long Qx = round_up(Q, GT);  // exponent tables: [kb][Qx] per block
@@ -1264,7 +1273,7 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
             for (int b = 0, i0 = 0; i0 < oo; ++b, i0 += bo) {
                 const int nb = std::min(bo, oo - i0);
                 rc = by_gslices<SliceRows>(SG, ctx, GB_GRAD, src + (long)i0 * vv, (long)oo * vv, Q, 1, nb * vv,
-                                          round_up(nb * vv, GB_GRAD), dst + (size_t)b * SG * Q * kwb,
+                                          grad_pitch(nb * vv), dst + (size_t)b * SG * Q * kwb,
                                           ex + (size_t)b * Qx * (kwb / GB_GRAD), Qx);
                 if (rc) return rc;
             }
@@ -1369,7 +1378,7 @@ int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& o, const double* 
     int rc;
     if (dBl) {
         // dBl[q][(i,a)] += sum_{(j,b)} Br[q][(j,b)] Z[(i,a)][(j,b)]
-        const long R = (long)nI * vl, kw = round_up(nJ * vr, GB_GRAD), Rx = round_up(R, GT), Qx = round_up(Q, GT);
+        const long R = (long)nI * vl, kw = grad_pitch(nJ * vr), Rx = round_up(R, GT), Qx = round_up(Q, GT);
         const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)Rx * (kw / GB_GRAD);
         char* w = static_cast<char*>(workspace(ctx, WS_ZS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;
@@ -1386,7 +1395,7 @@ int bf16_grad_contract(mshqc_hip_ctx* ctx, const Bf16Operands& o, const double* 
     }
     if (second && dBr) {
         // dBr[q][(j,b)] += sum_{(i,a)} Bl[q][(i,a)] Z[(i,a)][(j,b)]
-        const long R = (long)nJ * vr, kw = round_up(nI * vl, GB_GRAD), Rx = round_up(R, GT), Qx = round_up(Q, GT);
+        const long R = (long)nJ * vr, kw = grad_pitch(nI * vl), Rx = round_up(R, GT), Qx = round_up(Q, GT);
         const size_t zb = (size_t)S * R * kw * 2, ze = (size_t)Rx * (kw / GB_GRAD);
         char* w = static_cast<char*>(workspace(ctx, WS_ZTS, zb + ze + 256));
         if (!w) return MSHQC_HIP_ERR_ALLOC;

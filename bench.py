@@ -5,6 +5,10 @@ Workload (default, `mp2_o100`): closed-shell CD-RMP2 energy plus gradient
 B^Q_ia and orbital energies, inputs resident in HBM before timing starts.
 One step = one full E+grad evaluation.  The split-BF16 form (4 slices) is
 timed and reported; the FP64 form is timed beside it as the comparison.
+The RMP2 workloads also time the energy alone in both forms (`energy_only`:
+the pair kernel with its fused denominator epilogue, no amplitudes or
+gradient); at o=60, v=540, Q=2400 the FP64 energy is the plain form's
+output.
 
 Other workloads (`--workload`):
   ump2_o120  CD-UMP2 energy + gradient, o_a=120, v_a=1380, o_b=118,
@@ -260,6 +264,25 @@ def mp2_line(args, results, precs, W3, config, energy_of, grads_of):
     return line
 
 
+def energy_only_leg(args, torch, ctx, precs, dB, deo, dev, W):
+    """The CD-RMP2 energy alone (no amplitudes, no gradient: the pair kernel
+    with its fused denominator epilogue) in each timed precision, at most 5
+    steps after one warm-up, with the energy difference against FP64 when
+    both forms run."""
+    out = {}
+    for p in precs:
+        sec, E, prof = timed(torch, ctx, lambda: ctx.rmp2_energy(dB, deo, dev, precision=p),
+                             min(args.steps, 5), min(args.warmup, 1))
+        log(f"[bench] energy only {prec_name(p)}: {sec:.4f} s/step, E = {E!r}")
+        out[prec_name(p)] = {"seconds": sec, "energy": E, "tflops": W / sec / 1e12,
+                             "step_seconds": step_stats(prof["step_ms"])}
+    if 0 in precs and len(precs) > 1:
+        other = prec_name(precs[0] if precs[0] != 0 else precs[1])
+        out["abs_dE"] = abs(out[other]["energy"] - out["f64"]["energy"])
+        out["speedup"] = out["f64"]["seconds"] / out[other]["seconds"]
+    return out
+
+
 def run_mp2(args, torch, hip, syn, w):
     ctx = hip.context(0)
     if args.zblock:
@@ -317,6 +340,8 @@ def run_mp2(args, torch, hip, syn, w):
         results[p] = dict(sec=sec, out=out, prof=prof)
     config["precision"] = prec_name(precs[0])
     line = mp2_line(args, results, precs, W3, config, energy_of, grads_of)
+    if w["kind"] == "rmp2" and not args.no_energy_only:
+        line["energy_only"] = energy_only_leg(args, torch, ctx, precs, dB, deo, dev, W3 / 3)
     if w["kind"] == "rmp2" and not args.no_cpu:
         log(f"[bench] cpu baseline on {args.cpu_pairs} sampled pairs ({args.cpu_ref_pairs} for the reference loop)")
         line["cpu_baseline"] = cpu_baseline(B, eo, ev, args.cpu_pairs, args.cpu_ref_pairs)
@@ -600,6 +625,8 @@ def main():
                     help="cap (doubles) of the gradient amplitude block per occupied block pair "
                          "(0: the library default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-energy-only", action="store_true",
+                    help="skip the energy-only leg of the RMP2 workloads")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB moved by one plain device kernel before timing (a known-size "
                          "kernel for scaling PMC byte counters)")

@@ -226,9 +226,11 @@ def mp2_line(args, results, precs, W3, config, energy_of, grads_of):
     line["kernels"] = {
         "pair_kernel": {"launches_per_step": npair_launch / steps, "ms_per_step": pair_ms_step,
                         "algorithmic_flop_per_step": W, "hardware_products_per_flop": hw_factor},
-        "gemm": {"launches_per_step": ngemm / steps, "ms_per_step": gemm_ms_step,
+        "gemm": {"timed_calls_per_step": ngemm / steps, "ms_per_step": gemm_ms_step,
                  "algorithmic_flop_per_step": 2 * W, "hardware_products_per_flop": gemm_hw,
-                 "note": "includes the slicing of the amplitude block"},
+                 "note": "gradient contractions; a timed call is one GEMM in FP64 and one occupied "
+                         "block pair (its one or two GEMMs and the slicing of the amplitude block) "
+                         "in the split form"},
     }
     line["roofline"] = {
         "kernel": "pair-block (ia|jb) kernel with fused denominator epilogue",

@@ -152,6 +152,32 @@ def test_grad_blocks_bf16(ctx):
     assert abs(E - E_ref) / abs(E_ref) <= 1e-12
 
 
+@pytest.mark.parametrize("Q,o,v,bo", [(96, 8, 512, None), (64, 8, 1024, 4)])
+def test_grad_padded_pitch(ctx, Q, o, v, bo):
+    """Gradient operands whose rows would be a multiple of 8 KiB apart (an
+    occupied block's o_b * v columns: 4096 and 8192 BF16 values) carry one more
+    zero exponent block per row; the contractions must be unchanged by it.
+    The second case runs two occupied blocks of 4 (4096 columns each) and so
+    the off-diagonal block pair's transposed slicing too."""
+    from mshqc_amd import hip
+
+    B, eo, ev = syn.rmp2_inputs(Q, o, v, 84)
+    E_ref, g_ref, deo_ref, dev_ref = oracle.rmp2_grad(B, eo, ev)
+    c = hip.Context(0)
+    try:
+        if bo:
+            c.set_zblock((bo * v) ** 2)
+        E, g, d_o, d_v = c.rmp2_grad(*to_dev(B, eo, ev), precision=4)
+    finally:
+        c.close()
+    rel = np.linalg.norm(to_np(g) - g_ref) / np.linalg.norm(g_ref)
+    print(f"Q={Q} o={o} v={v} bo={bo}: grad rel {rel:.2e}, E rel {abs(E - E_ref) / abs(E_ref):.2e}", flush=True)
+    assert rel <= 1e-6
+    assert abs(E - E_ref) / abs(E_ref) <= E_TOL[4]
+    check_close(to_np(d_o), deo_ref, 1e-6, "dE/deps_o")
+    check_close(to_np(d_v), dev_ref, 1e-6, "dE/deps_v")
+
+
 def test_ump2_bf16(ctx):
     (Ba, eoa, eva), (Bb, eob, evb) = syn.ump2_inputs(160, 4, 70, 3, 75, 83)
     E_ref, grads_ref = oracle.ump2(Ba, Bb, eoa, eva, eob, evb, grad=True)

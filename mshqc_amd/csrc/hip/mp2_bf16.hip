@@ -224,8 +224,18 @@ __global__ __launch_bounds__(256) void slice_cols_kernel(const double* __restric
     __shared__ double T[GB][NC + 1];
     __shared__ int exs[NC];
     const int ntv = vp / GB;
-    const int c0 = blockIdx.x * NC;
-    const int ip = blockIdx.y / ntv, at = blockIdx.y % ntv, a0 = at * GB;
+    // Work item w = (a block, column tile), column tiles fastest.  The
+    // dispatcher deals workgroups to the 8 XCDs in turn; remapped so that
+    // each XCD takes a contiguous range of w, the workgroups that read the
+    // two halves of a 128-byte line of Z (adjacent column tiles) run on one
+    // XCD and share its L2.  The grid is a multiple of 8; items past the end
+    // return (whole workgroups, before any barrier).
+    const int nct = (nc + NC - 1) / NC;
+    const long w = (long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    if (w >= (long)nct * nI * ntv) return;
+    const int c0 = (int)(w % nct) * NC;
+    const int rb = (int)(w / nct);
+    const int ip = rb / ntv, at = rb % ntv, a0 = at * GB;
     const int tid = threadIdx.x;
     {
         const int cc = tid % NC;
@@ -1076,8 +1086,9 @@ struct SliceCols {
     template <int GB>
     static void launch(mshqc_hip_ctx* ctx, const double* Z, long ldz, int nI, int v, int vp, int nc,
                        uint16_t* dst, int8_t* ex, long ldx) {
-        dim3 grid(ceil_div(nc, sc_cols(GB)), nI * (vp / GB));
-        slice_cols_kernel<S, GB><<<grid, 256, 0, ctx->stream>>>(Z, ldz, nI, v, vp, nc, dst, ex, ldx);
+        const long items = (long)ceil_div(nc, sc_cols(GB)) * nI * (vp / GB);
+        slice_cols_kernel<S, GB><<<(unsigned)((items + 7) / 8 * 8), 256, 0, ctx->stream>>>(Z, ldz, nI, v, vp, nc,
+                                                                                     dst, ex, ldx);
     }
     static int run(mshqc_hip_ctx* ctx, int gb, const double* Z, long ldz, int nI, int v, int vp, int nc,
                    uint16_t* dst, int8_t* ex, long ldx) {

@@ -19,7 +19,7 @@ struct Bf16Operands {
     int spb = 8;      // pair-kernel stages (of 16 Q) per exponent block, 8 or 10
     int ol = 0, vl = 0, orr = 0, vr = 0;
     int vEl = 0, vEr = 0;  // row stride of the pair-kernel exponent tables
-    // Pair-kernel operands: slices [Qp/16][ceil(o*v/32)][S][32][16] (bf16 bits) and
+    // Pair-kernel operands: slices [Qp/16][ceil(o*v/32) + 1][S][32][16] (bf16 bits) and
     // exponents [Qp/(16 spb)][o][vE], left and right (right = left in the
     // symmetric modes).
     const uint16_t* sl = nullptr;

@@ -1,6 +1,6 @@
 // Split-BF16 form of the MP2 kernels on gfx950 BF16 matrix cores.
 //
-//   slice_pair_kernel   B[Q][o*v] -> BF16 slices [Qp/16][ceil(o*v/32)][S][32][16] (each
+//   slice_pair_kernel   B[Q][o*v] -> BF16 slices [Qp/16][ceil(o*v/32) + 1][S][32][16] (each
 //                       16-wide Q step of 32 rows is one contiguous KiB) and
 //                       one exponent per (i, a) and 128-wide Q block
 //   pair_bf16_kernel    (ia|jb) tiles from the slices on 32x32x16 MFMAs, exact
@@ -80,11 +80,15 @@ inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
 // ------------------------------------------------------------- slicing
 
-// B[Q][o*v] (row-major, rows = q) -> Ps[Qp/16][ceil(o*v/32)][S][32][16] and
+// B[Q][o*v] (row-major, rows = q) -> Ps[Qp/16][ceil(o*v/32) + 1][S][32][16] and
 // Pe[Qp/QB][o][vE] (exponent of row (i, a) over one QB-wide Q block; the
 // caller zeroes Pe, so the padding a >= v reads 0).  One workgroup per 32
 // rows x QB q; the tile is transposed through LDS (dynamic, QB x 33 doubles)
-// so both the FP64 reads and the BF16 writes are contiguous.
+// so both the FP64 reads and the BF16 writes are contiguous.  Rows past o*v,
+// including the whole extra 32-row block, hold zero slices: the pair kernel
+// feeds that block to the MFMAs of padded virtual indices, whose outputs it
+// discards.  Zero operands draw less power than the next orbital's rows did,
+// and the pair kernel runs power-limited (1.8-1.9 GHz).
 constexpr size_t slice_pair_lds(int QB) { return (size_t)QB * 33 * sizeof(double); }
 
 template <int S, int QB>
@@ -124,7 +128,6 @@ __global__ __launch_bounds__(256) void slice_pair_kernel(const double* __restric
     // block of one 16-wide Q step.
     for (int idx = tid; idx < (QB / KS) * 32 * 2; idx += 256) {
         const int kq = idx >> 6, row = (idx & 63) >> 1, half = idx & 1;
-        if (r0 + row >= ov) continue;
         const int e = ex[row];
         uint16_t out[S][8];
 #pragma unroll
@@ -492,31 +495,31 @@ __global__ __launch_bounds__(256) void pair_bf16_kernel(PairArgs p, const uint16
         const int blk = diag ? (k < 2 ? 2 * wv + k : 2 * wv) : BPW * wv + k;
         rdst[k] = blk * 1024;
         bool left;
-        long r0;
+        int va;  // virtual index of the block's first row in its orbital
         if (SYM && !diag) {
             left = blk < 2 || blk >= 8;
-            r0 = blk < 2   ? (long)i * p.vl + R0 + 32 * blk
-                 : blk < 6 ? (long)j * p.vr + C0 + 32 * (blk - 2)
-                 : blk < 8 ? (long)j * p.vr + R0 + 32 * (blk - 6)
-                           : (long)i * p.vl + C0 + 32 * (blk - 8);
-        } else if (SYM) {
-            // diagonal block: 0-3 Bl_i rows, 4-7 Br_j rows
-            left = blk < 4;
-            r0 = blk < 4 ? (long)i * p.vl + R0 + 32 * blk : (long)j * p.vr + C0 + 32 * (blk - 4);
+            va = blk < 2 ? R0 + 32 * blk : blk < 6 ? C0 + 32 * (blk - 2) : blk < 8 ? R0 + 32 * (blk - 6)
+                                                                                   : C0 + 32 * (blk - 8);
         } else {
+            // diagonal block (symmetric) or opposite-spin unit: 0-3 Bl_i
+            // rows, 4-7 Br_j rows
             left = blk < 4;
-            r0 = blk < 4 ? (long)i * p.vl + R0 + 32 * blk : (long)j * p.vr + C0 + 32 * (blk - 4);
+            va = blk < 4 ? R0 + 32 * blk : C0 + 32 * (blk - 4);
         }
         const long rows = left ? ovl : ovr;
-        const long row = min(r0 + lr, rows - 1);
+        const long r0 = (long)(left ? i : j) * (left ? p.vl : p.vr) + va;
+        // Rows past the orbital's virtual count (padding of the last tile)
+        // read the all-zero 32-row block after the operand's last, so the
+        // MFMAs whose outputs are discarded run on zeros.
+        const long row = va + lr < (left ? p.vl : p.vr) ? r0 + lr : (rows + 31) / 32 * 32 + lr;
         rleft[k] = left;
         roff[k] = (unsigned)(2 * (((row >> 5) * S * 32 + (row & 31)) * KS + 8 * lseg));
     }
-    // Slices are [Qp/16][ceil(o*v/32)][S][32][16]: the S slices of one
+    // Slices are [Qp/16][ceil(o*v/32) + 1][S][32][16]: the S slices of one
     // 32-row block of a 16-Q stage are adjacent KiBs.  A load addresses its
     // stage by the resource base, its row by the lane offset and its slice by
     // a constant scalar offset.
-    const long sstepl = (long)S * 32 * KS * ((ovl + 31) / 32), sstepr = (long)S * 32 * KS * ((ovr + 31) / 32);
+    const long sstepl = (long)S * 32 * KS * ((ovl + 31) / 32 + 1), sstepr = (long)S * 32 * KS * ((ovr + 31) / 32 + 1);
     // One slice of a stage; a stage's loads are spread over its MFMA groups
     // (a burst of them stalls instruction issue behind the address unit).
     const int nst = Qp / KS;
@@ -1050,7 +1053,7 @@ struct SlicePair {
     static void launch(mshqc_hip_ctx* ctx, const double* B, int Q, int o, int v, int vE, int Qp, uint16_t* Ps,
                        int8_t* Pe) {
         const int ov = o * v;
-        dim3 grid(ceil_div(ov, 32), Qp / QB);
+        dim3 grid(ceil_div(ov, 32) + 1, Qp / QB);
         slice_pair_kernel<S, QB><<<grid, 256, slice_pair_lds(QB), ctx->stream>>>(B, Q, ov, v, vE, o, Qp, Ps, Pe);
     }
     static int run(mshqc_hip_ctx* ctx, int spb, const double* B, int Q, int o, int v, int vE, int Qp,
@@ -1216,16 +1219,16 @@ int bf16_prepare(mshqc_hip_ctx* ctx, int slices, const double* Bl, int ol, int v
     o.qblocks = o.Qp / QB;
     const long ovl = (long)ol * vl, ovr = Br ? (long)orr * vr : 0;
     // The pair kernel's buffer loads address one 16-Q stage block
-    // ([ceil(o*v/32)][S][32][16]) by 32-bit byte offsets.
-    if ((long)S * (std::max(ovl, ovr) + 31) >= (1L << 27))
+    // ([ceil(o*v/32) + 1][S][32][16]) by 32-bit byte offsets.
+    if ((long)S * (std::max(ovl, ovr) + 63) >= (1L << 27))
         return set_error(ctx, MSHQC_HIP_ERR_ARG,
                          "split-BF16: a stage block of %d slices x o*v = %ld rows exceeds the pair kernel's 4 GiB", S,
                          std::max(ovl, ovr));
-    // Pair-kernel slices [Qp/16][ceil(o*v/32)][S][32][16] and exponent tables
+    // Pair-kernel slices [Qp/16][ceil(o*v/32) + 1][S][32][16] and exponent tables
     // [Qp/QB][o][vE], vE = v rounded up to 128 (zero padding).
     o.vEl = round_up(vl, 128);
     o.vEr = Br ? round_up(vr, 128) : o.vEl;
-    const size_t bl = (size_t)S * ((ovl + 31) / 32 * 32) * o.Qp * 2, br = (size_t)S * ((ovr + 31) / 32 * 32) * o.Qp * 2;
+    const size_t bl = (size_t)S * ((ovl + 31) / 32 * 32 + 32) * o.Qp * 2, br = (size_t)S * ((ovr + 31) / 32 * 32 + 32) * o.Qp * 2;
     const size_t el = (size_t)o.qblocks * ol * o.vEl, er = Br ? (size_t)o.qblocks * orr * o.vEr : 0;
     // self terms after the exponent tables, 256-byte aligned
     const int nso = self_terms ? (Br ? std::min(ol, orr) : ol) : 0;

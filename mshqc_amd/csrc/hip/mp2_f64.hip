@@ -117,8 +117,12 @@ __global__ __launch_bounds__(NT) void pair_f64_kernel(PairArgs p) {
 // q*512 + 16 * ((c/2) ^ (8 * (q & 1))) + 8 * (c & 1): the 16 lanes of a
 // fragment read touch 8 segments of one q row, the next 16 lanes the other
 // half-bank group of the next row, so reads are free of bank conflicts.
-// Columns past the operand's virtual count read a clamped in-row pair (their
-// outputs are discarded); rows q >= Q of the last stage are zeroed.
+// Columns past the operand's virtual count read zeros (their outputs are
+// discarded); rows q >= Q of the last stage are zeroed.
+
+// 16 bytes of zeros (never written): the DMA source of the padded columns.
+__device__ __align__(16) double pair_zero_seg[2];
+
 __device__ __forceinline__ int pf_off(int q, int c) {
     return q * 512 + ((((c >> 1) ^ ((q & 1) << 3))) << 4) + ((c & 1) << 3);
 }
@@ -166,8 +170,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         const int v = left ? p.vl : p.vr;
         const int c0 = (P == 0 || P == 2) ? A0 : B0;
         const double* base = left ? p.Bl + (long)i * p.vl : p.Br + (long)j * p.vr;
-        col_ptr[k] = base + min(c0 + 2 * cp, v - 2);
-        qstride[k] = left ? (long)p.ol * p.vl : (long)p.orr * p.vr;
+        // Columns past the virtual count (v is even, so a segment is whole
+        // or wholly past) read a fixed 16-byte zero segment at every q: their
+        // outputs are discarded, and zero operands draw less power than real
+        // columns would.
+        const bool pad = c0 + 2 * cp >= v;
+        col_ptr[k] = pad ? pair_zero_seg : base + c0 + 2 * cp;
+        qstride[k] = pad ? 0 : left ? (long)p.ol * p.vl : (long)p.orr * p.vr;
         qrow[k] = qr;
         dst[k] = P * PANEL + uu * 1024;
     }

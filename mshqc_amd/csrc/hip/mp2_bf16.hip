@@ -1095,6 +1095,9 @@ struct SliceCols {
     }
     static int run(mshqc_hip_ctx* ctx, int gb, const double* Z, long ldz, int nI, int v, int vp, int nc,
                    uint16_t* dst, int8_t* ex, long ldx) {
+        // one workgroup per (a block, column tile), 1-D grid
+        if ((long)ceil_div(nc, sc_cols(gb)) * nI * (vp / gb) > 0x7ffffff8L)
+            return set_error(ctx, MSHQC_HIP_ERR_ARG, "split-BF16 transposed slicing: grid too large");
         if (gb == GB_GRAD) launch<GB_GRAD>(ctx, Z, ldz, nI, v, vp, nc, dst, ex, ldx);
         else launch<GB_TRANSFORM>(ctx, Z, ldz, nI, v, vp, nc, dst, ex, ldx);
         LAUNCH_CHECK(ctx);
